@@ -1,0 +1,550 @@
+// extern "C" boundary (include/aspher_hip.h): exceptions become status codes
+// plus a thread-local message; device state stays behind opaque handles.
+#include "../../include/aspher_hip.h"
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+using namespace aspher;
+
+struct aspher_solver {
+    std::unique_ptr<Context> ctx;
+    std::shared_ptr<Operator> op;
+    std::string backend;
+    int Ns = 0;
+};
+
+struct aspher_result {
+    Result r;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        g_err.clear();
+        f();
+        return ASPHER_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return ASPHER_ERR_INVALID;
+    } catch (const std::logic_error& e) {
+        g_err = e.what();
+        return ASPHER_ERR_LOGIC;
+    } catch (const DeviceError& e) {
+        g_err = e.what();
+        return ASPHER_ERR_DEVICE;
+    } catch (const std::bad_alloc& e) {
+        g_err = "host allocation failed";
+        return ASPHER_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return ASPHER_ERR_RUNTIME;
+    }
+}
+
+void copy_str(char* dst, std::size_t n, const std::string& s) {
+    std::strncpy(dst, s.c_str(), n - 1);
+    dst[n - 1] = '\0';
+}
+
+Precond::Kind precond_kind(const std::string& name) {
+    // python/bindings.cpp:193-214
+    if (name == "fourier") return Precond::Kind::stencil;
+    if (name == "fourier-fft") return Precond::Kind::fourier;
+    if (name == "none" || name.empty()) return Precond::Kind::none;
+    throw std::invalid_argument("precond must be 'none', 'fourier' or 'fourier-fft'");
+}
+
+NestedParams to_params(const aspher_nested_params* p) {
+    NestedParams np;
+    if (!p) return np;
+    np.coarsest = p->coarsest;
+    np.q = p->q;
+    np.leaf_side = p->leaf_side;
+    np.precond = p->precond != 0;
+    np.precond_engine = p->precond_engine;
+    np.precond_radius = p->precond_radius;
+    np.precond_occupancy_max = p->precond_occupancy_max;
+    np.precond_cost_gate = p->precond_cost_gate;
+    np.coarse_tol = p->coarse_tol;
+    np.single_precision = p->single_precision != 0;
+    np.precision = p->precision;
+    np.float_floor = p->float_floor;
+    np.allow_tolerance_relaxation = p->allow_tolerance_relaxation != 0;
+    np.light_result = p->light_result != 0;
+    np.backend = p->backend;
+    np.record_error_history = p->record_error_history != 0;
+    np.active_set = p->active_set != 0;
+    np.active_delta = p->active_delta;
+    np.active_halo = p->active_halo;
+    np.active_max_rounds = p->active_max_rounds;
+    np.active_all_levels = p->active_all_levels != 0;
+    np.active_occupancy_max = p->active_occupancy_max;
+    return np;
+}
+
+void solve_single(aspher_solver* s, const double* gap_dev, double p_nominal, double tol,
+                  int max_iter, int use_pr, const char* precond, const double* p_init_dev,
+                  aspher_result** out) {
+    if (!s || !out) throw std::invalid_argument("null handle");
+    const Precond::Kind pk = precond_kind(precond ? precond : "none");
+    Precond pc(pk, s->Ns, 2);
+    // solve_contact defaults (include/contact_solver.hpp:151-191)
+    SolveOptions opt;
+    opt.tol = tol;
+    opt.max_iter = max_iter;
+    opt.use_pr = use_pr != 0;
+    auto res = std::make_unique<aspher_result>();
+    const std::int64_t N = (std::int64_t)s->Ns * s->Ns;
+    DevBuf<double> p0;
+    if (p_init_dev) {
+        p0.alloc(N);
+        VecOps<double>::copy(N, p_init_dev, p0.p, s->ctx->stream);
+    }
+    solve_impl<double>(*s->ctx, *s->op, gap_dev, p_nominal, opt, &pc,
+                       p_init_dev ? p0.p : nullptr, res->r);
+    *out = res.release();
+}
+
+} // namespace
+
+extern "C" {
+
+const char* aspher_last_error(void) { return g_err.c_str(); }
+
+const char* aspher_version(void) { return "aspher-mi355x 0.1.0 (gfx950)"; }
+
+int aspher_device_count(int* count) {
+    return guard([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        if (count) *count = n;
+    });
+}
+
+int aspher_solver_create(int grid_size, double domain_size, double E_star, const char* backend,
+                         aspher_solver** out) {
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null output handle");
+        // validation order of python/bindings.cpp:122-127
+        if (grid_size <= 0) throw std::invalid_argument("grid_size must be positive");
+        if (!(domain_size > 0.0)) throw std::invalid_argument("domain_size must be positive");
+        if (!(E_star > 0.0)) throw std::invalid_argument("E_star must be positive");
+        const std::string b = backend ? backend : "";
+        Operator::Kind kind;
+        if (b == "fft") kind = Operator::Kind::free_space;
+        else if (b == "periodic") kind = Operator::Kind::periodic;
+        else if (b == "dense") kind = Operator::Kind::dense;
+        else if (b == "h2" || b == "hmatrix" || b.empty())
+            throw std::invalid_argument("backend '" + (b.empty() ? std::string("hmatrix") : b) +
+                                        "' is not part of this build (expected fft, periodic "
+                                        "or dense)");
+        else
+            throw std::invalid_argument("unknown backend: " + b +
+                                        " (expected dense, hmatrix, h2, fft, or periodic)");
+        auto s = std::make_unique<aspher_solver>();
+        s->ctx = std::make_unique<Context>();
+        s->op = std::make_shared<Operator>(kind, grid_size, domain_size, E_star, s->ctx->stream);
+        s->op->build();
+        s->backend = b;
+        s->Ns = grid_size;
+        *out = s.release();
+    });
+}
+
+void aspher_solver_destroy(aspher_solver* s) {
+    if (!s) return;
+    try {
+        s->op.reset();
+        delete s;
+    } catch (...) {
+    }
+}
+
+int aspher_solver_matvec(aspher_solver* s, const double* p, double* u) {
+    return guard([&] {
+        if (!s || !p || !u) throw std::invalid_argument("null pointer");
+        const std::int64_t N = (std::int64_t)s->Ns * s->Ns;
+        DevBuf<double> x(N);
+        hipStream_t st = s->ctx->stream;
+        ASPHER_HIP(hipMemcpyAsync(x.p, p, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        s->op->apply(x.p, x.p, st);
+        ASPHER_HIP(hipMemcpyAsync(u, x.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int precision) {
+    return guard([&] {
+        if (!s || !x || !y) throw std::invalid_argument("null pointer");
+        ASPHER_HIP(hipDeviceSynchronize());
+        hipStream_t st = s->ctx->stream;
+        if (precision == 64) {
+            s->op->apply(static_cast<const double*>(x), static_cast<double*>(y), st);
+        } else if (precision == 32) {
+            s->op->build_single();
+            s->op->apply(static_cast<const float*>(x), static_cast<float*>(y), st);
+        } else {
+            throw std::invalid_argument("precision must be 64 or 32");
+        }
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_solver_time_matvec(aspher_solver* s, const void* x, void* y, int precision, int reps,
+                              double* ms) {
+    return guard([&] {
+        if (!s || !x || !y || !ms || reps <= 0) throw std::invalid_argument("bad arguments");
+        ASPHER_HIP(hipDeviceSynchronize());
+        hipStream_t st = s->ctx->stream;
+        if (precision == 32) s->op->build_single();
+        auto run = [&] {
+            if (precision == 64)
+                s->op->apply(static_cast<const double*>(x), static_cast<double*>(y), st);
+            else
+                s->op->apply(static_cast<const float*>(x), static_cast<float*>(y), st);
+        };
+        run();  // warm-up
+        hipEvent_t a, b;
+        ASPHER_HIP(hipEventCreate(&a));
+        ASPHER_HIP(hipEventCreate(&b));
+        ASPHER_HIP(hipEventRecord(a, st));
+        for (int i = 0; i < reps; ++i) run();
+        ASPHER_HIP(hipEventRecord(b, st));
+        ASPHER_HIP(hipEventSynchronize(b));
+        float t = 0.0f;
+        ASPHER_HIP(hipEventElapsedTime(&t, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *ms = t / reps;
+    });
+}
+
+int aspher_solver_solve(aspher_solver* s, const double* gap, double p_nominal, double tol,
+                        int max_iter, int use_pr, const char* precond, const double* p_init,
+                        aspher_result** out) {
+    return guard([&] {
+        if (!s || !gap) throw std::invalid_argument("null pointer");
+        const std::int64_t N = (std::int64_t)s->Ns * s->Ns;
+        hipStream_t st = s->ctx->stream;
+        (void)precond_kind(precond ? precond : "none");
+        DevBuf<double> g(N), p0;
+        ASPHER_HIP(hipMemcpyAsync(g.p, gap, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        if (p_init) {
+            p0.alloc(N);
+            ASPHER_HIP(hipMemcpyAsync(p0.p, p_init, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        }
+        solve_single(s, g.p, p_nominal, tol, max_iter, use_pr, precond, p_init ? p0.p : nullptr,
+                     out);
+    });
+}
+
+int aspher_solver_solve_device(aspher_solver* s, const double* gap_dev, double p_nominal,
+                               double tol, int max_iter, int use_pr, const char* precond,
+                               const double* p_init_dev, aspher_result** out) {
+    return guard([&] {
+        if (!s || !gap_dev) throw std::invalid_argument("null pointer");
+        ASPHER_HIP(hipDeviceSynchronize());
+        solve_single(s, gap_dev, p_nominal, tol, max_iter, use_pr, precond, p_init_dev, out);
+    });
+}
+
+int aspher_solver_info(aspher_solver* s, aspher_operator_info* out) {
+    return guard([&] {
+        if (!s || !out) throw std::invalid_argument("null pointer");
+        std::memset(out, 0, sizeof(*out));
+        copy_str(out->backend, sizeof(out->backend), s->backend);
+        out->n = (int64_t)s->Ns * s->Ns;
+        out->padded_side = s->backend == "fft" ? 2 * s->Ns : s->Ns;
+        out->bytes_spectrum = (int64_t)s->op->bytes_spectrum();
+        out->bytes_scratch = (int64_t)s->op->bytes_scratch();
+        out->bytes = out->bytes_spectrum + out->bytes_scratch;
+        out->build_seconds = s->op->build_seconds;
+    });
+}
+
+void aspher_nested_params_default(aspher_nested_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->coarsest = 64;
+    p->q = 6;
+    p->leaf_side = 8;
+    p->precond = 1;
+    std::strcpy(p->precond_engine, "auto");
+    p->precond_radius = 2;
+    p->precond_occupancy_max = 0.4;
+    p->precond_cost_gate = 0.0;
+    p->coarse_tol = 1e-4;
+    p->single_precision = 0;
+    p->precision[0] = '\0';
+    p->float_floor = 2e-6;
+    p->allow_tolerance_relaxation = 0;
+    p->light_result = 0;
+    std::strcpy(p->backend, "h2");
+    p->record_error_history = 0;
+    p->active_set = 0;
+    p->active_delta = 0.05;
+    p->active_halo = 2;
+    p->active_max_rounds = 5;
+    p->active_all_levels = 1;
+    p->active_occupancy_max = 0.4;
+    p->reuse_operators = 0;
+}
+
+int aspher_solve_nested(int grid_size, const double* gap, double p_nominal, double domain_size,
+                        double E_star, double tol, int max_iter, int use_pr,
+                        const aspher_nested_params* params, aspher_result** out) {
+    return guard([&] {
+        if (!gap || !out) throw std::invalid_argument("null pointer");
+        if (grid_size <= 0)
+            throw std::invalid_argument("solve_contact_nested: Ns must be positive");
+        const std::int64_t N = (std::int64_t)grid_size * grid_size;
+        DevBuf<double> g(N);
+        ASPHER_HIP(hipMemcpy(g.p, gap, sizeof(double) * N, hipMemcpyHostToDevice));
+        auto res = std::make_unique<aspher_result>();
+        const NestedParams np = to_params(params);
+        solve_nested(grid_size, domain_size, E_star, g.p, p_nominal, tol, max_iter, use_pr != 0,
+                     np, res->r, params && params->reuse_operators != 0);
+        *out = res.release();
+    });
+}
+
+int aspher_solve_nested_device(int grid_size, const double* gap_dev, double p_nominal,
+                               double domain_size, double E_star, double tol, int max_iter,
+                               int use_pr, const aspher_nested_params* params,
+                               aspher_result** out) {
+    return guard([&] {
+        if (!gap_dev || !out) throw std::invalid_argument("null pointer");
+        ASPHER_HIP(hipDeviceSynchronize());
+        auto res = std::make_unique<aspher_result>();
+        const NestedParams np = to_params(params);
+        solve_nested(grid_size, domain_size, E_star, gap_dev, p_nominal, tol, max_iter,
+                     use_pr != 0, np, res->r, params && params->reuse_operators != 0);
+        *out = res.release();
+    });
+}
+
+int aspher_result_get_info(const aspher_result* rr, aspher_result_info* o) {
+    return guard([&] {
+        if (!rr || !o) throw std::invalid_argument("null pointer");
+        const Result& r = rr->r;
+        std::memset(o, 0, sizeof(*o));
+        o->grid_size = r.Ns;
+        o->objective = r.objective;
+        o->contact_area = r.contact_fraction;
+        o->mean_pressure = r.mean_pressure;
+        o->approach = r.approach;
+        o->iterations = r.iterations;
+        o->error = r.error;
+        o->converged = r.converged ? 1 : 0;
+        o->active_rounds = r.active_rounds;
+        o->active_fallback = r.active_fallback ? 1 : 0;
+        copy_str(o->status, sizeof(o->status), to_string(r.status));
+        copy_str(o->status_reason, sizeof(o->status_reason), r.status_reason);
+        o->pk_error = r.pk_error;
+        o->fw_gap = r.fw_gap;
+        o->fw_error = r.fw_error;
+        o->load_error = r.load_error;
+        o->pressure_violation = r.pressure_violation;
+        o->penetration_error = r.penetration_error;
+        o->requested_tol = r.requested_tol;
+        o->effective_tol = r.effective_tol;
+        o->p_ref = r.p_ref;
+        o->g_ref = r.g_ref;
+        copy_str(o->validation_scope, sizeof(o->validation_scope), r.validation_scope);
+        copy_str(o->operator_error_kind, sizeof(o->operator_error_kind), r.operator_error_kind);
+        o->matvec_count = r.matvec_count;
+        o->verification_matvec_count = r.verification_matvec_count;
+        o->precond_count = r.precond_count;
+        o->identification_steps = r.identification_steps;
+        o->returned_best = r.returned_best ? 1 : 0;
+        o->precond_dropped = r.precond_dropped ? 1 : 0;
+        o->mem_cg_state = r.mem_cg_state;
+        o->mem_best_iterate = r.mem_best;
+        o->mem_contact_mask = r.mem_mask;
+        o->mem_output_arrays = r.mem_output;
+        o->mem_peak = r.mem_peak;
+        o->t_total = r.time_total;
+        o->t_matvec = r.time_matvec;
+        o->t_precond = r.time_precond;
+        o->t_build = r.time_build;
+        o->t_coarse = r.time_coarse;
+        o->t_verification = r.time_verification;
+        o->t_candidate = r.time_candidate;
+        o->t_output = r.time_output;
+        o->has_displacement = r.displacement.p ? 1 : 0;
+        o->has_gap = r.gap.p ? 1 : 0;
+        o->n_history = (int)r.error_history.size();
+        o->n_stages = (int)r.stage_stats.size();
+    });
+}
+
+int aspher_result_field_device(const aspher_result* rr, int which, const double** dev) {
+    return guard([&] {
+        if (!rr || !dev) throw std::invalid_argument("null pointer");
+        const Result& r = rr->r;
+        const DevBuf<double>* b = which == 0 ? &r.pressure : which == 1 ? &r.displacement
+                                  : which == 2 ? &r.gap : nullptr;
+        if (!b) throw std::invalid_argument("field index must be 0, 1 or 2");
+        if (!b->p) throw std::invalid_argument("field not stored (light result)");
+        *dev = b->p;
+    });
+}
+
+int aspher_result_get_field(const aspher_result* rr, int which, double* out) {
+    return guard([&] {
+        if (!rr || !out) throw std::invalid_argument("null pointer");
+        const Result& r = rr->r;
+        const DevBuf<double>* b = which == 0 ? &r.pressure : which == 1 ? &r.displacement
+                                  : which == 2 ? &r.gap : nullptr;
+        if (!b) throw std::invalid_argument("field index must be 0, 1 or 2");
+        if (!b->p) throw std::invalid_argument("field not stored (light result)");
+        ASPHER_HIP(hipMemcpy(out, b->p, b->bytes(), hipMemcpyDeviceToHost));
+    });
+}
+
+int aspher_result_get_history(const aspher_result* rr, double* out) {
+    return guard([&] {
+        if (!rr || !out) throw std::invalid_argument("null pointer");
+        const auto& h = rr->r.error_history;
+        if (!h.empty()) std::memcpy(out, h.data(), sizeof(double) * h.size());
+    });
+}
+
+int aspher_result_get_stage(const aspher_result* rr, int index, aspher_stage_info* o) {
+    return guard([&] {
+        if (!rr || !o) throw std::invalid_argument("null pointer");
+        const auto& st = rr->r.stage_stats;
+        if (index < 0 || index >= (int)st.size()) throw std::invalid_argument("stage index");
+        const Stage& s = st[index];
+        std::memset(o, 0, sizeof(*o));
+        copy_str(o->name, sizeof(o->name), s.name);
+        copy_str(o->precision, sizeof(o->precision), s.precision);
+        o->q = s.q;
+        o->requested_tol = s.requested_tol;
+        o->effective_tol = s.effective_tol;
+        o->iterations = s.iterations;
+        o->matvec_count = s.matvec_count;
+        o->seconds = s.seconds;
+        copy_str(o->status, sizeof(o->status), to_string(s.status));
+        o->fw_error = s.fw_error;
+        o->penetration_error = s.penetration_error;
+        o->precond_dropped = s.precond_dropped ? 1 : 0;
+    });
+}
+
+void aspher_result_destroy(aspher_result* r) {
+    try {
+        delete r;
+    } catch (...) {
+    }
+}
+
+int aspher_configure_allocator(int64_t, int64_t, int* applied) {
+    if (applied) *applied = 0;
+    return ASPHER_OK;
+}
+
+int aspher_love_points(int n, const double* x, const double* y, const double* a,
+                       const double* b, double* out) {
+    return guard([&] {
+        if (n <= 0) return;
+        DevBuf<double> d(5 * (std::size_t)n);
+        ASPHER_HIP(hipMemcpy(d.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+        ASPHER_HIP(hipMemcpy(d.p + n, y, sizeof(double) * n, hipMemcpyHostToDevice));
+        ASPHER_HIP(hipMemcpy(d.p + 2 * n, a, sizeof(double) * n, hipMemcpyHostToDevice));
+        ASPHER_HIP(hipMemcpy(d.p + 3 * n, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        love_points(n, d.p, d.p + n, d.p + 2 * n, d.p + 3 * n, d.p + 4 * n, nullptr);
+        ASPHER_HIP(hipMemcpy(out, d.p + 4 * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int aspher_love_table(int grid_size, double domain_size, double E_star, double* out) {
+    return guard([&] {
+        if (grid_size <= 0) throw std::invalid_argument("grid_size must be positive");
+        const std::int64_t N = (std::int64_t)grid_size * grid_size;
+        DevBuf<double> t(N);
+        love_table(grid_size, domain_size, E_star, t.p, nullptr);
+        ASPHER_HIP(hipMemcpy(out, t.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    });
+}
+
+int aspher_stencil_taps(int grid_size, int radius, int* ntaps, int* dx, int* dy, double* w) {
+    return guard([&] {
+        if (radius < 1) throw std::invalid_argument("stencil radius must be >= 1");
+        if (2 * radius + 1 > grid_size)
+            throw std::invalid_argument("stencil radius too large for the grid");
+        std::vector<int> vx, vy;
+        std::vector<double> vw;
+        stencil_weights(grid_size, radius, vx, vy, vw);
+        *ntaps = (int)vw.size();
+        for (std::size_t t = 0; t < vw.size(); ++t) {
+            dx[t] = vx[t];
+            dy[t] = vy[t];
+            w[t] = vw[t];
+        }
+    });
+}
+
+} // extern "C"
+
+namespace {
+template <typename T>
+void precond_apply_t(int Ns, Precond::Kind kind, int radius, const double* g,
+                     const uint8_t* mask, double* z) {
+    Context ctx;
+    hipStream_t st = ctx.stream;
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    DevBuf<double> gd(N);
+    DevBuf<T> gt(N), zt(N);
+    DevBuf<uint8_t> m(N);
+    ASPHER_HIP(hipMemcpyAsync(gd.p, g, sizeof(double) * N, hipMemcpyHostToDevice, st));
+    ASPHER_HIP(hipMemcpyAsync(m.p, mask, N, hipMemcpyHostToDevice, st));
+    VecOps<T>::from_double(N, gd.p, 0.0, gt.p, st);
+    const int nblk = vec_blocks(N);
+    static const int ops[2] = {RSUM, RSUM};
+    static const int dst[2] = {S_R0, S_NC};
+    VecOps<T>::masked_sum(N, gt.p, m.p, ctx.part.p, nblk, st);
+    finalize<T>(ctx.part.p, nblk, 2, ops, dst, ctx.scal.p, STEP_NONE, st);
+    ctx.read_scal();
+    const double nc = ctx.host_scal[S_NC];
+    const int idx[1] = {S_GMEAN};
+    const double val[1] = {nc > 0 ? ctx.host_scal[S_R0] / nc : 0.0};
+    set_scal(ctx.scal.p, 1, idx, val, st);
+    Precond pc(kind, Ns, radius);
+    pc.apply<T>(gt.p, m.p, zt.p, ctx.scal.p, ctx.part.p, st);
+    VecOps<T>::masked_sub(N, zt.p, m.p, ctx.scal.p, S_ZMEAN, st);
+    VecOps<T>::to_double(N, zt.p, gd.p, st);
+    ASPHER_HIP(hipMemcpyAsync(z, gd.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    ASPHER_HIP(hipStreamSynchronize(st));
+}
+} // namespace
+
+extern "C" {
+
+int aspher_precond_apply(int grid_size, const char* kind, int radius, int precision,
+                         const double* g, const uint8_t* mask, double* z) {
+    return guard([&] {
+        if (!g || !mask || !z) throw std::invalid_argument("null pointer");
+        const Precond::Kind k = precond_kind(kind ? kind : "");
+        if (k == Precond::Kind::none) throw std::invalid_argument("no preconditioner named");
+        if (precision == 64) precond_apply_t<double>(grid_size, k, radius, g, mask, z);
+        else if (precision == 32) precond_apply_t<float>(grid_size, k, radius, g, mask, z);
+        else throw std::invalid_argument("precision must be 64 or 32");
+    });
+}
+
+int aspher_clear_cache(void) {
+    return guard([&] { clear_operator_cache(); });
+}
+
+} // extern "C"

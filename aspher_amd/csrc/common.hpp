@@ -1,0 +1,74 @@
+// Shared host/device definitions for the MI355X spectral contact path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace aspher {
+
+// HIP status -> exception. The C-ABI layer catches and maps it to a code.
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define ASPHER_HIP(call)                                                        \
+    do {                                                                        \
+        hipError_t err__ = (call);                                              \
+        if (err__ != hipSuccess)                                                \
+            throw ::aspher::DeviceError(std::string(#call) + ": " +             \
+                                        hipGetErrorString(err__) + " (" +       \
+                                        __FILE__ + ":" +                        \
+                                        std::to_string(__LINE__) + ")");        \
+    } while (0)
+
+#define ASPHER_LAUNCH_CHECK() ASPHER_HIP(hipGetLastError())
+
+// Interleaved complex value; alignment lets the compiler emit 8/16-byte
+// vector loads and stores for one element.
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T re, im;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> scale(cx<T> a, T s) {
+    return {a.re * s, a.im * s};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> conjg(cx<T> a) {
+    return {a.re, -a.im};
+}
+
+// Number of 256-thread blocks for grid-stride O(N) passes with block-level
+// reductions: enough to fill 256 CUs several times over, bounded so the
+// partial-sum buffer stays small and the final reduction is one block.
+constexpr int kVecThreads = 256;
+constexpr int kMaxVecBlocks = 2048;
+// partial-sum buffer: one row of up to kPartVals values per block
+constexpr int kPartVals = 8;
+// rows of the partial buffer of one solve context (>= any reduction's block count)
+constexpr int kMaxPartRows = 16384;
+
+inline int vec_blocks(std::int64_t n) {
+    std::int64_t b = (n + 4LL * kVecThreads - 1) / (4LL * kVecThreads);
+    if (b < 1) b = 1;
+    if (b > kMaxVecBlocks) b = kMaxVecBlocks;
+    return static_cast<int>(b);
+}
+
+} // namespace aspher

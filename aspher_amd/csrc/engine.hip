@@ -1,0 +1,331 @@
+// Host engine of the spectral contact path (see engine.hpp).
+//
+// The projected-CG state machine follows src/contact_solver.cpp:150-587 step
+// by step; every O(N) loop is a device pass (kernels.hip), every operator or
+// preconditioner application a spectral pipeline (spectral.hip). The host
+// reads the FP64 diagnostic block once per iteration. The CG update of an
+// iteration is launched without waiting for its line-search verdict: the
+// verdict (positive curvature or not) is read together with the next
+// iteration's diagnostics. When the verdict is negative the update kernels
+// were predicated off on the device, the pressure is unchanged, and the
+// speculatively re-evaluated state is bit-identical to the state the
+// reference hands to its identification step, so the control flow is the
+// reference's exactly.
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
+
+namespace aspher {
+
+const char* to_string(Status s) {
+    switch (s) {
+        case Status::converged: return "converged";
+        case Status::stagnated: return "stagnated";
+        case Status::max_iterations: return "max_iterations";
+        case Status::nonpositive_curvature: return "nonpositive_curvature";
+        case Status::nonfinite: return "nonfinite";
+        case Status::verification_failed: return "verification_failed";
+        case Status::resource_limit: return "resource_limit";
+    }
+    return "unknown";
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+inline double since(Clock::time_point t0) {
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+const int kSum1[1] = {RSUM};
+
+} // namespace
+
+// ── Context ────────────────────────────────────────────────────────────────
+
+Context::Context() {
+    ASPHER_HIP(hipStreamCreate(&stream));
+    scal.alloc(S_COUNT);
+    part.alloc((std::size_t)kMaxPartRows * kPartVals);
+    ASPHER_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_scal), sizeof(double) * S_COUNT));
+    ASPHER_HIP(hipMemsetAsync(scal.p, 0, sizeof(double) * S_COUNT, stream));
+}
+
+Context::~Context() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (host_scal) (void)hipHostFree(host_scal);
+    scal.reset();
+    part.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::read_scal() {
+    ASPHER_HIP(hipMemcpyAsync(host_scal, scal.p, sizeof(double) * S_COUNT,
+                              hipMemcpyDeviceToHost, stream));
+    ASPHER_HIP(hipStreamSynchronize(stream));
+}
+
+// ── Operator ───────────────────────────────────────────────────────────────
+
+Operator::Operator(Kind kind, int Ns, double L, double E_star, hipStream_t s)
+    : kind_(kind), Ns_(Ns), L_(L), E_(E_star), stream_(s) {
+    if (Ns <= 0) throw std::invalid_argument("grid_size must be positive");
+    if (!(L > 0.0)) throw std::invalid_argument("domain_size must be positive");
+    if (!(E_star > 0.0)) throw std::invalid_argument("E_star must be positive");
+    if (kind == Kind::periodic && Ns < 2)
+        throw std::invalid_argument("grid_size must be at least 2");
+}
+
+Operator::~Operator() = default;
+
+void Operator::build() {
+    const auto t0 = Clock::now();
+    const std::int64_t N = (std::int64_t)Ns_ * Ns_;
+    if (kind_ == Kind::free_space) {
+        // FP64 Love table, then the forward transform of its (2Ns)^2
+        // wrap-around embedding; K is real and even, so only the (Ns+1)^2
+        // unique values Re/M^2 are kept (src/fft_operator.cpp:18-69).
+        const int M = 2 * Ns_;
+        DevBuf<double> table(N);
+        love_table(Ns_, L_, E_, table.p, stream_);
+        kh64_.alloc((std::size_t)(Ns_ + 1) * (Ns_ + 1));
+        {
+            SpectralPlan<double> bplan(M, M, M, 0, 0);
+            DevBuf<double> part((std::size_t)2 * bplan.col_blocks());
+            bplan.build_spectrum(table.p, Ns_, kh64_.p, part.p, stream_);
+            std::vector<double> h(part.n);
+            ASPHER_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes(), hipMemcpyDeviceToHost,
+                                      stream_));
+            ASPHER_HIP(hipStreamSynchronize(stream_));
+            double mi = 0.0, mr = 0.0;
+            for (std::size_t b = 0; b < h.size() / 2; ++b) {
+                mi = std::max(mi, h[2 * b]);
+                mr = std::max(mr, h[2 * b + 1]);
+            }
+            if (mi > 1e-10 * mr)
+                throw std::runtime_error(
+                    "FFTOperator: kernel spectrum has a non-negligible imaginary part; "
+                    "wrap-around embedding is broken");
+        }
+        plan64_ = std::make_unique<SpectralPlan<double>>(M, Ns_, Ns_, Ns_, Ns_);
+    } else if (kind_ == Kind::periodic) {
+        plan64_ = std::make_unique<SpectralPlan<double>>(Ns_, Ns_, Ns_, Ns_, Ns_);
+    } else {
+        table_.alloc(N);
+        love_table(Ns_, L_, E_, table_.p, stream_);
+    }
+    ASPHER_HIP(hipStreamSynchronize(stream_));
+    built_ = true;
+    build_seconds = since(t0);
+}
+
+void Operator::build_single() {
+    if (!built_) throw std::logic_error("FFTOperator::build_single_caches: build() not called");
+    if (kind_ == Kind::free_space && !kh32_.p) {
+        kh32_.alloc(kh64_.n);
+        cast_d2f((std::int64_t)kh64_.n, kh64_.p, kh32_.p, stream_);
+    }
+    if (kind_ != Kind::dense && !plan32_) {
+        const int n = kind_ == Kind::free_space ? 2 * Ns_ : Ns_;
+        plan32_ = std::make_unique<SpectralPlan<float>>(n, Ns_, Ns_, Ns_, Ns_);
+    }
+}
+
+void Operator::release_single() {
+    plan32_.reset();
+    kh32_.reset();
+    tmp32_.reset();
+}
+
+template <>
+SpectralPlan<double>& Operator::plan<double>() {
+    return *plan64_;
+}
+template <>
+SpectralPlan<float>& Operator::plan<float>() {
+    if (!plan32_) build_single();
+    return *plan32_;
+}
+
+template <typename T>
+void Operator::apply_t(const T* x, T* y, hipStream_t s) {
+    if (!built_) throw std::logic_error("FFTOperator::matvec: build() not called");
+    if (kind_ == Kind::free_space) {
+        const T* kh;
+        if constexpr (std::is_same_v<T, double>) kh = kh64_.p;
+        else {
+            if (!kh32_.p) build_single();
+            kh = kh32_.p;
+        }
+        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::table, kh, Ns_ + 1, 0.0, s);
+    } else if (kind_ == Kind::periodic) {
+        // q = 2 pi k/L, u_hat = 2 p_hat/(E*|q|), unnormalised pair -> 1/Ns^2
+        // (src/periodic_fft_operator.cpp:11-24)
+        const double pi = std::acos(-1.0);
+        const double sc = L_ / (pi * E_ * static_cast<double>(Ns_) * Ns_);
+        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s);
+    } else {
+        const std::int64_t N = (std::int64_t)Ns_ * Ns_;
+        if (x == y) {
+            DevBuf<T>& tmp = [&]() -> DevBuf<T>& {
+                if constexpr (std::is_same_v<T, double>) return tmp64_;
+                else return tmp32_;
+            }();
+            if (tmp.n != (std::size_t)N) tmp.alloc(N);
+            VecOps<T>::copy(N, x, tmp.p, s);
+            VecOps<T>::dense(Ns_, table_.p, tmp.p, y, s);
+        } else {
+            VecOps<T>::dense(Ns_, table_.p, x, y, s);
+        }
+    }
+}
+
+void Operator::apply(const double* x, double* y, hipStream_t s) { apply_t<double>(x, y, s); }
+void Operator::apply(const float* x, float* y, hipStream_t s) { apply_t<float>(x, y, s); }
+
+std::size_t Operator::bytes_spectrum() const {
+    return kh64_.bytes() + kh32_.bytes();
+}
+std::size_t Operator::bytes_scratch() const {
+    std::size_t b = 0;
+    if (plan64_) b += plan64_->workspace_bytes();
+    if (plan32_) b += plan32_->workspace_bytes();
+    return b;
+}
+
+// ── Preconditioners ────────────────────────────────────────────────────────
+
+void stencil_weights(int Ns, int R, std::vector<int>& dx, std::vector<int>& dy,
+                     std::vector<double>& w) {
+    // w = F^-1 |k| on Nw = min(Ns, 512) (src/stencil_precond.cpp:19-43,
+    // 199-220): the half spectrum C(kx, ky) = hypot(kx, kof(ky)), DC zeroed,
+    // inverse c2c along ky then c2r along kx (Im of the DC and Nyquist
+    // columns ignored), evaluated directly at the (2R+1)^2 needed offsets.
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::tuple<std::vector<int>, std::vector<int>,
+                                                    std::vector<double>>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(Ns, R);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        std::tie(dx, dy, w) = it->second;
+        return;
+    }
+    const int Nw = Ns < 512 ? Ns : 512;
+    const int nh = Nw / 2 + 1;
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    std::vector<long double> cs(Nw), sn(Nw);
+    for (int m = 0; m < Nw; ++m) {
+        cs[m] = cosl(two_pi * m / Nw);
+        sn[m] = sinl(two_pi * m / Nw);
+    }
+    auto kof = [Nw](int i) { return (i < Nw / 2) ? i : i - Nw; };
+    // the offsets actually needed, as grid indices
+    std::vector<int> need;
+    for (int d = -R; d <= R; ++d) need.push_back(((d % Nw) + Nw) % Nw);
+    std::sort(need.begin(), need.end());
+    need.erase(std::unique(need.begin(), need.end()), need.end());
+    auto G = [&](int ix, int iy) -> long double {
+        long double acc = 0.0L;
+        for (int kx = 0; kx < nh; ++kx) {
+            // B(kx, iy) = sum_ky C(kx, ky) exp(+2 pi i ky iy / Nw)
+            long double br = 0.0L, bi = 0.0L;
+            for (int ky = 0; ky < Nw; ++ky) {
+                if (kx == 0 && ky == 0) continue;
+                const long double c = std::hypot((double)kx, (double)kof(ky));
+                const int m = (int)(((long long)ky * iy) % Nw);
+                br += c * cs[m];
+                bi += c * sn[m];
+            }
+            const bool edge = (kx == 0) || (Nw % 2 == 0 && kx == Nw / 2);
+            if (edge) {
+                const int m = (int)(((long long)kx * ix) % Nw);
+                acc += br * cs[m];
+            } else {
+                const int m = (int)(((long long)kx * ix) % Nw);
+                acc += 2.0L * (br * cs[m] - bi * sn[m]);
+            }
+        }
+        return acc;
+    };
+    std::map<std::pair<int, int>, long double> val;
+    for (int iy : need)
+        for (int ix : need) val[{ix, iy}] = G(ix, iy);
+    const long double c0 = val[{0, 0}];
+    dx.clear();
+    dy.clear();
+    w.clear();
+    for (int ddy = -R; ddy <= R; ++ddy)
+        for (int ddx = -R; ddx <= R; ++ddx) {
+            if (ddx * ddx + ddy * ddy > R * R) continue;
+            const int wx = ((ddx % Nw) + Nw) % Nw, wy = ((ddy % Nw) + Nw) % Nw;
+            dx.push_back(ddx);
+            dy.push_back(ddy);
+            w.push_back((ddx == 0 && ddy == 0) ? 1.0 : (double)(val[{wx, wy}] / c0));
+        }
+    cache[key] = std::make_tuple(dx, dy, w);
+}
+
+Precond::Precond(Kind kind, int Ns, int radius) : kind_(kind), Ns_(Ns) {
+    if (kind_ == Kind::stencil) {
+        if (radius < 1) throw std::invalid_argument("stencil radius must be >= 1");
+        if (2 * radius + 1 > Ns)
+            throw std::invalid_argument("stencil radius too large for the grid");
+        stencil_weights(Ns, radius, dx_, dy_, w_);
+        std::vector<int> d(2 * dx_.size());
+        for (std::size_t t = 0; t < dx_.size(); ++t) {
+            d[2 * t] = dx_[t];
+            d[2 * t + 1] = dy_[t];
+        }
+        tapd_.alloc(d.size());
+        tapw_.alloc(w_.size());
+        ASPHER_HIP(hipMemcpy(tapd_.p, d.data(), sizeof(int) * d.size(), hipMemcpyHostToDevice));
+        ASPHER_HIP(hipMemcpy(tapw_.p, w_.data(), sizeof(double) * w_.size(),
+                             hipMemcpyHostToDevice));
+    } else if (kind_ == Kind::fourier) {
+        // |k|/Ns^2 in float (src/fourier_precond.cpp:12-28)
+        norm_ = 1.0f / (static_cast<float>(Ns) * static_cast<float>(Ns));
+    }
+}
+
+template <typename T>
+void Precond::apply(const T* g, const std::uint8_t* mask, T* z, double* scal, double* part,
+                    hipStream_t s) {
+    static const int dz[1] = {S_ZSUM};
+    if (kind_ == Kind::stencil) {
+        const int nblk = vec_blocks((std::int64_t)Ns_ * Ns_);
+        VecOps<T>::stencil(Ns_, g, mask, scal, tapd_.p, tapw_.p, (int)w_.size(), z, part,
+                           nblk, s);
+        finalize<T>(part, nblk, 1, kSum1, dz, scal, STEP_ZMEAN, s);
+    } else if (kind_ == Kind::fourier) {
+        SpectralPlan<T>* pl;
+        if constexpr (std::is_same_v<T, double>) {
+            if (!plan64_) plan64_ = std::make_unique<SpectralPlan<double>>(Ns_, Ns_, Ns_, Ns_, Ns_);
+            pl = plan64_.get();
+        } else {
+            if (!plan32_) plan32_ = std::make_unique<SpectralPlan<float>>(Ns_, Ns_, Ns_, Ns_, Ns_);
+            pl = plan32_.get();
+        }
+        if (pl->row_out_blocks() > kMaxPartRows)
+            throw std::invalid_argument("grid too large for the preconditioner reduction");
+        pl->convolve(g, Ns_, z, Ns_, Symbol::precond, nullptr, 0, (double)norm_, s,
+                     RowIn::masked, RowOut::masked, mask, scal, S_GMEAN, part);
+        finalize<T>(part, pl->row_out_blocks(), 1, kSum1, dz, scal, STEP_ZMEAN, s);
+    }
+}
+
+template void Precond::apply<double>(const double*, const std::uint8_t*, double*, double*,
+                                     double*, hipStream_t);
+template void Precond::apply<float>(const float*, const std::uint8_t*, float*, double*,
+                                    double*, hipStream_t);
+
+} // namespace aspher

@@ -1,0 +1,253 @@
+// C++17 host side of the spectral contact path: operators, preconditioners,
+// the certified Polonsky–Keer state machine and the nested-grid driver. All
+// vectors live on the device for the whole solve; the host reads one block
+// of FP64 diagnostics per iteration.
+#pragma once
+
+#include "common.hpp"
+#include "spectral.hpp"
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace aspher {
+
+// RAII device allocation.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(std::size_t count) { alloc(count); }
+    ~DevBuf() { reset(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    void alloc(std::size_t count) {
+        reset();
+        if (count) ASPHER_HIP(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count));
+        n = count;
+    }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    std::size_t bytes() const { return sizeof(T) * n; }
+};
+
+// Termination state (include/contact_solver.hpp:13-21).
+enum class Status : int {
+    converged = 0, stagnated, max_iterations, nonpositive_curvature, nonfinite,
+    verification_failed, resource_limit,
+};
+const char* to_string(Status s);
+
+// Solver policy (include/contact_solver.hpp:134-191).
+struct SolveOptions {
+    double tol = 1e-8;
+    double requested_tol = 0.0;
+    int max_iter = 5000;
+    bool use_pr = true;
+    bool light = false;
+    bool record_history = false;
+    bool keep_best = true;
+    double load_tol = 0.0;
+    int stall_limit = 200;
+    double precond_cost_gate = 0.0;
+    bool allow_tolerance_relaxation = false;
+    double p_ref = 0.0;
+    double g_ref = 0.0;
+    enum class Datum { automatic, solver, caller } datum_mode = Datum::automatic;
+    double datum = 0.0;
+};
+
+struct Stage {
+    std::string name, precision;
+    int q = 0;
+    double requested_tol = 0.0, effective_tol = 0.0;
+    int iterations = 0;
+    long long matvec_count = 0;
+    double seconds = 0.0;
+    Status status = Status::max_iterations;
+    double fw_error = 0.0, penetration_error = 0.0;
+    bool precond_dropped = false;
+};
+
+// ContactResult (include/contact_solver.hpp:25-148). Fields stay on the
+// device (FP64) until the caller copies them out.
+struct Result {
+    int Ns = 0;
+    DevBuf<double> pressure, displacement, gap;
+    double approach = 0.0, objective = 0.0, error = 0.0;
+    int iterations = 0;
+    bool converged = false;
+    double contact_fraction = 0.0, mean_pressure = 0.0;
+    std::vector<double> error_history;
+    int active_rounds = 0;
+    bool active_fallback = false;
+    Status status = Status::max_iterations;
+    std::string status_reason;
+    double pk_error = 0.0, fw_gap = 0.0, fw_error = 0.0, load_error = 0.0;
+    double pressure_violation = 0.0, penetration_error = 0.0;
+    double requested_tol = 0.0, effective_tol = 0.0, p_ref = 0.0, g_ref = 0.0;
+    std::string validation_scope = "solve_operator";
+    std::string operator_error_kind = "unavailable";
+    long long mem_cg_state = 0, mem_best = 0, mem_mask = 0, mem_output = 0, mem_peak = 0;
+    double time_total = 0.0, time_matvec = 0.0, time_precond = 0.0, time_build = 0.0;
+    double time_coarse = 0.0, time_verification = 0.0, time_candidate = 0.0,
+           time_output = 0.0;
+    std::vector<Stage> stage_stats;
+    long long matvec_count = 0, verification_matvec_count = 0, precond_count = 0;
+    int identification_steps = 0;
+    bool returned_best = false, precond_dropped = false;
+};
+
+// Influence operator on the device: free-space (padded Love convolution),
+// doubly periodic (2/(E*|q|)) or dense (direct Love sum, small grids).
+class Operator {
+public:
+    enum class Kind { free_space, periodic, dense };
+    Operator(Kind kind, int Ns, double L, double E_star, hipStream_t s);
+    ~Operator();
+    Operator(const Operator&) = delete;
+    Operator& operator=(const Operator&) = delete;
+
+    // free space: FP64 Love table + FP64 padded r2c of its embedding,
+    // realness check, (Ns+1)^2 real spectrum (src/fft_operator.cpp:18-69)
+    void build();
+    void build_single();    // FP32 spectrum copy + FP32 plan
+    void release_single();
+    bool built() const { return built_; }
+
+    // y = S x on device buffers (x, y may alias for the spectral kinds)
+    void apply(const double* x, double* y, hipStream_t s);
+    void apply(const float* x, float* y, hipStream_t s);
+
+    Kind kind() const { return kind_; }
+    int Ns() const { return Ns_; }
+    double L() const { return L_; }
+    double E() const { return E_; }
+    std::size_t bytes_spectrum() const;
+    std::size_t bytes_scratch() const;
+    double build_seconds = 0.0;
+    // device Love table (dense kind; free space keeps it only if asked)
+    const double* table() const { return table_.p; }
+
+private:
+    template <typename T> void apply_t(const T* x, T* y, hipStream_t s);
+    template <typename T> SpectralPlan<T>& plan();
+
+    Kind kind_;
+    int Ns_;
+    double L_, E_;
+    hipStream_t stream_;
+    bool built_ = false;
+    DevBuf<double> table_;
+    DevBuf<double> kh64_;      // (Ns+1) x (Ns+1), [kx][ky]
+    DevBuf<float> kh32_;
+    std::unique_ptr<SpectralPlan<double>> plan64_;
+    std::unique_ptr<SpectralPlan<float>> plan32_;
+    DevBuf<double> tmp64_;     // dense aliasing scratch
+    DevBuf<float> tmp32_;
+};
+
+// Preconditioner: none, 13-tap stencil (src/stencil_precond.cpp) or full-grid
+// Fourier |k| (src/fourier_precond.cpp).
+class Precond {
+public:
+    enum class Kind { none, stencil, fourier };
+    Precond(Kind kind, int Ns, int radius = 2);
+    Kind kind() const { return kind_; }
+    // z = M^-1 g on the mask (mean-zeroed); uses scal[S_GMEAN] and leaves
+    // scal[S_ZMEAN] for the consumer. Returns false for `none`.
+    template <typename T>
+    void apply(const T* g, const std::uint8_t* mask, T* z, double* scal, double* part,
+               hipStream_t s);
+    const std::vector<int>& tap_dx() const { return dx_; }
+    const std::vector<int>& tap_dy() const { return dy_; }
+    const std::vector<double>& tap_w() const { return w_; }
+
+private:
+    Kind kind_;
+    int Ns_;
+    std::vector<int> dx_, dy_;
+    std::vector<double> w_;
+    DevBuf<int> tapd_;
+    DevBuf<double> tapw_;
+    std::unique_ptr<SpectralPlan<double>> plan64_;
+    std::unique_ptr<SpectralPlan<float>> plan32_;
+    float norm_ = 0.0f;
+};
+
+// Host-side stencil weights: w = F^-1|k| on an Nw x Nw grid (DC zeroed),
+// normalised by w(0,0), sampled on the l2 disc of radius R.
+void stencil_weights(int Ns, int R, std::vector<int>& dx, std::vector<int>& dy,
+                     std::vector<double>& w);
+
+// Per-call device context: stream + reusable scalar/partial buffers.
+struct Context {
+    hipStream_t stream = nullptr;
+    DevBuf<double> scal, part;
+    double* host_scal = nullptr;   // pinned mirror
+    Context();
+    ~Context();
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    void read_scal();              // D2H of the scalar block + stream sync
+};
+
+// One certified PK solve (src/contact_solver.cpp:150-587) on device inputs:
+// g0 (N values of Real), optional p_init (consumed, may be modified).
+template <typename Real>
+void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
+                const SolveOptions& opt, Precond* pc, Real* p_init, Result& res);
+
+// Nested-grid driver parameters (include/nested_solve.hpp:11-119; the fft
+// and periodic branches).
+struct NestedParams {
+    int coarsest = 64;
+    int q = 6;
+    int leaf_side = 8;
+    bool precond = true;
+    std::string precond_engine = "auto";   // auto | stencil | fft
+    int precond_radius = 2;
+    double precond_occupancy_max = 0.4;
+    double precond_cost_gate = 0.0;
+    double coarse_tol = 1e-4;
+    std::string precision = "";            // double | float | float_then_double | ""
+    bool single_precision = false;
+    double float_floor = 2e-6;
+    bool allow_tolerance_relaxation = false;
+    bool light_result = false;
+    std::string backend = "h2";
+    bool record_error_history = false;
+    bool active_set = false;
+    double active_delta = 0.05;
+    int active_halo = 2;
+    int active_max_rounds = 5;
+    bool active_all_levels = true;
+    double active_occupancy_max = 0.4;
+};
+
+bool stencil_for_level(const std::string& engine, bool precond, double prev_occupancy,
+                       double occupancy_max);
+
+// gap: device FP64 Ns*Ns. Operators are built per level (or taken from the
+// process-wide cache when `reuse_operators` is on).
+void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double p_bar,
+                  double tol, int max_iter, bool use_pr, const NestedParams& np,
+                  Result& res, bool reuse_operators = false);
+
+// Process-wide operator cache (kind, Ns, L, E) -> built operator.
+std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L, double E,
+                                          hipStream_t s);
+void clear_operator_cache();
+
+} // namespace aspher

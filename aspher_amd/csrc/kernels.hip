@@ -1,0 +1,807 @@
+// O(N) kernels of the certified projected CG (restating the loops of
+// src/contact_solver.cpp:150-587 as fused device passes with FP64
+// accumulation), the FP64 Love table (src/boussinesq_kernel.cpp:44-106), the
+// dense operator and the nested-grid transfers (src/nested_solve.cpp:42-71).
+//
+// Every reduction writes one partial row per block (grid-stride loop in a
+// fixed order, fixed shuffle tree) and is finished by `finalize`, a single
+// block reducing the partials in index order: deterministic, bit-reproducible
+// results. The finalize steps compute the derived scalars (approach, beta,
+// step lengths, ...) on the device so the host only reads the diagnostics.
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+#include <cmath>
+
+namespace aspher {
+
+namespace {
+
+#define GRID_LOOP(i, N)                                                          \
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x;   \
+         i < (N); i += (std::int64_t)gridDim.x * blockDim.x)
+
+template <int OP>
+__device__ __forceinline__ void put(double v, double* red, double* part, int nv, int idx) {
+    v = block_reduce<OP>(v, red);
+    if (threadIdx.x == 0) part[(std::size_t)blockIdx.x * kPartVals + idx] = v;
+    (void)nv;
+}
+
+// ── evaluate (src/contact_solver.cpp:260-303) ──────────────────────────────
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_eval1(std::int64_t N, const T* __restrict__ u, const T* __restrict__ g0s,
+        const T* __restrict__ p, T* __restrict__ g, double* __restrict__ part) {
+    __shared__ double red[8];
+    double gsum = 0.0, nc = 0.0, vmin = __builtin_huge_val();
+    GRID_LOOP(i, N) {
+        const T v = u[i] + g0s[i];
+        g[i] = v;
+        const double vd = (double)v;
+        if (vd < vmin) vmin = vd;
+        if (p[i] > T(0)) {
+            gsum += vd;
+            nc += 1.0;
+        }
+    }
+    put<RSUM>(gsum, red, part, 3, 0);
+    put<RSUM>(nc, red, part, 3, 1);
+    put<RMIN>(vmin, red, part, 3, 2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
+        std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
+        double* __restrict__ part) {
+    __shared__ double red[8];
+    const T alpha = static_cast<T>(scal[S_ALPHA]);
+    const T gmin = static_cast<T>(scal[S_GMIN]);
+    double pv = 0.0, gcs = 0.0, e = 0.0, Gc = 0.0, psum = 0.0;
+    GRID_LOOP(i, N) {
+        const T pi = p[i];
+        const double pd = (double)pi;
+        T gi = g[i];
+        pv += pd * (double)gi;
+        gi = gi - alpha;
+        g[i] = gi;
+        const double gd = (double)gi;
+        const bool c = pi > T(0);
+        if (c) gcs += gd;
+        e += pd * fabs(gd);
+        Gc += pd * (double)(T)(gi - gmin);
+        psum += pd;
+        mask[i] = c ? 1 : 0;
+    }
+    put<RSUM>(pv, red, part, 5, 0);
+    put<RSUM>(gcs, red, part, 5, 1);
+    put<RSUM>(e, red, part, 5, 2);
+    put<RSUM>(Gc, red, part, 5, 3);
+    put<RSUM>(psum, red, part, 5, 4);
+}
+
+// ── stencil preconditioner apply_full (src/stencil_precond.cpp:63-109) ────
+// 2-D tiles: a 64 x 4 block of outputs per workgroup; taps read through L1.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask,
+          const double* __restrict__ scal, const int* __restrict__ tapd,
+          const double* __restrict__ tapw, int ntaps, T* __restrict__ z,
+          double* __restrict__ part) {
+    __shared__ double red[8];
+    const double gmean = scal[S_GMEAN];
+    double zsum = 0.0;
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    GRID_LOOP(i, N) {
+        const int iy = (int)(i / Ns), ix = (int)(i - (std::int64_t)iy * Ns);
+        if (!mask[i]) {
+            z[i] = T(0);
+            continue;
+        }
+        double acc = 0.0;
+        for (int t = 0; t < ntaps; ++t) {
+            int jx = ix + tapd[2 * t], jy = iy + tapd[2 * t + 1];
+            if (jx < 0) jx += Ns; else if (jx >= Ns) jx -= Ns;
+            if (jy < 0) jy += Ns; else if (jy >= Ns) jy -= Ns;
+            const std::int64_t j = (std::int64_t)jy * Ns + jx;
+            const double v = mask[j] ? (double)g[j] - gmean : 0.0;
+            acc += tapw[t] * v;
+        }
+        z[i] = static_cast<T>(acc);
+        zsum += acc;
+    }
+    put<RSUM>(zsum, red, part, 1, 0);
+}
+
+// ── CG direction (src/contact_solver.cpp:411-440) ──────────────────────────
+
+template <typename T, bool PC>
+__global__ void __launch_bounds__(256)
+k_cgdot(std::int64_t N, const T* __restrict__ g, const T* __restrict__ gprev,
+        const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
+        const double* __restrict__ scal, double* __restrict__ part) {
+    __shared__ double red[8];
+    const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
+    double G = 0.0, Gpr = 0.0, gg = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        const T gi = g[i];
+        const T zi = PC ? T(z[i] - zmean) : gi;
+        const double gd = (double)gi, zd = (double)zi;
+        G += zd * gd;
+        Gpr += zd * (double)(T)(gi - gprev[i]);
+        gg += gd * gd;
+    }
+    put<RSUM>(G, red, part, 3, 0);
+    put<RSUM>(Gpr, red, part, 3, 1);
+    put<RSUM>(gg, red, part, 3, 2);
+}
+
+template <typename T, bool PC>
+__global__ void __launch_bounds__(256)
+k_tupdate(std::int64_t N, const T* __restrict__ g, T* __restrict__ gprev,
+          const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
+          T* __restrict__ t, const double* __restrict__ scal) {
+    const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
+    const T beta = static_cast<T>(scal[S_BETA]);
+    GRID_LOOP(i, N) {
+        const T gi = g[i];
+        if (mask[i]) {
+            const T zi = PC ? T(z[i] - zmean) : gi;
+            t[i] = zi + beta * t[i];
+        } else {
+            t[i] = T(0);
+        }
+        gprev[i] = gi;
+    }
+}
+
+// ── exact line search (src/contact_solver.cpp:446-459) ─────────────────────
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_line1(std::int64_t N, const T* __restrict__ r, const T* __restrict__ g,
+        const T* __restrict__ t, const std::uint8_t* __restrict__ mask,
+        double* __restrict__ part) {
+    __shared__ double red[8];
+    double rsum = 0.0, num = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        rsum += (double)r[i];
+        num += (double)g[i] * (double)t[i];
+    }
+    put<RSUM>(rsum, red, part, 2, 0);
+    put<RSUM>(num, red, part, 2, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_line2(std::int64_t N, const T* __restrict__ r, const T* __restrict__ t,
+        const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
+        double* __restrict__ part) {
+    __shared__ double red[8];
+    const T rmean = static_cast<T>(scal[S_RMEAN]);
+    double den = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        den += (double)(T)(r[i] - rmean) * (double)t[i];
+    }
+    put<RSUM>(den, red, part, 1, 0);
+}
+
+// ── update, overlap step, load rescale (src/contact_solver.cpp:477-501) ────
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_update(std::int64_t N, T* __restrict__ p, const T* __restrict__ t,
+         const T* __restrict__ g, const double* __restrict__ scal,
+         double* __restrict__ part) {
+    __shared__ double red[8];
+    if (scal[S_OK] == 0.0) {
+        if (threadIdx.x == 0) {
+            part[(std::size_t)blockIdx.x * kPartVals + 0] = 0.0;
+            part[(std::size_t)blockIdx.x * kPartVals + 1] = 0.0;
+        }
+        return;
+    }
+    const T tau = static_cast<T>(scal[S_TAU]);
+    const T tau_ov = static_cast<T>(scal[S_TAUOV]);
+    double changed = 0.0, total = 0.0;
+    GRID_LOOP(i, N) {
+        T pi = p[i];
+        const bool was = pi > T(0);
+        if (was) {
+            const T q = pi - tau * t[i];
+            pi = (q < T(0)) ? T(0) : q;  // std::max(q, 0): NaN propagates
+        }
+        const T gi = g[i];
+        if (pi == T(0) && gi < T(0)) pi = pi - tau_ov * gi;
+        if ((pi > T(0)) != was) changed = 1.0;
+        p[i] = pi;
+        total += (double)pi;
+    }
+    put<RMAX>(changed, red, part, 2, 0);
+    put<RSUM>(total, red, part, 2, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rescale(std::int64_t N, T* __restrict__ p, const double* __restrict__ scal) {
+    if (scal[S_OK] == 0.0) return;
+    const bool fill = scal[S_PFILL] != 0.0;
+    const T f = static_cast<T>(scal[S_F]);
+    const T pbar = static_cast<T>(scal[S_PBAR]);
+    GRID_LOOP(i, N) p[i] = fill ? pbar : T(p[i] * f);
+}
+
+// ── identification step and simplex projection (src/contact_solver.cpp:31-60,314-364)
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_objective(std::int64_t N, const T* __restrict__ p, const T* __restrict__ u,
+            const T* __restrict__ g0s, double* __restrict__ part) {
+    __shared__ double red[8];
+    double f = 0.0, pp = 0.0, pu = 0.0;
+    GRID_LOOP(i, N) {
+        const double pd = (double)p[i], ud = (double)u[i];
+        f += pd * (0.5 * ud + (double)g0s[i]);
+        pp += pd * pd;
+        pu += pd * ud;
+    }
+    put<RSUM>(f, red, part, 3, 0);
+    put<RSUM>(pp, red, part, 3, 1);
+    put<RSUM>(pu, red, part, 3, 2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_axpy_id(std::int64_t N, const T* __restrict__ p, const T* __restrict__ g,
+          T* __restrict__ z, T s) {
+    GRID_LOOP(i, N) z[i] = p[i] - s * g[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sum(std::int64_t N, const T* __restrict__ y, double* __restrict__ part) {
+    __shared__ double red[8];
+    double s = 0.0;
+    GRID_LOOP(i, N) s += (double)y[i];
+    put<RSUM>(s, red, part, 1, 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_count_above(std::int64_t N, const T* __restrict__ y, double theta,
+              double* __restrict__ part) {
+    __shared__ double red[8];
+    double s = 0.0, c = 0.0;
+    GRID_LOOP(i, N) {
+        const double v = (double)y[i];
+        if (v > theta) {
+            s += v;
+            c += 1.0;
+        }
+    }
+    put<RSUM>(s, red, part, 2, 0);
+    put<RSUM>(c, red, part, 2, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_project(std::int64_t N, T* __restrict__ y, T th) {
+    GRID_LOOP(i, N) {
+        const T v = y[i];
+        y[i] = (v > th) ? T(v - th) : T(0);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_idcheck(std::int64_t N, const T* __restrict__ z, const T* __restrict__ r,
+          const T* __restrict__ g0s, const T* __restrict__ g, const T* __restrict__ p,
+          double* __restrict__ part) {
+    __shared__ double red[8];
+    double f = 0.0, gd = 0.0;
+    GRID_LOOP(i, N) {
+        const double zd = (double)z[i];
+        f += zd * (0.5 * (double)r[i] + (double)g0s[i]);
+        gd += (double)g[i] * (double)(T)(z[i] - p[i]);
+    }
+    put<RSUM>(f, red, part, 2, 0);
+    put<RSUM>(gd, red, part, 2, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_stats(std::int64_t N, const T* __restrict__ y, double* __restrict__ part) {
+    __shared__ double red[8];
+    double mn = __builtin_huge_val(), mx = -__builtin_huge_val(), bad = 0.0;
+    GRID_LOOP(i, N) {
+        const double v = (double)y[i];
+        if (!isfinite(v)) bad += 1.0;
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    put<RMIN>(mn, red, part, 3, 0);
+    put<RMAX>(mx, red, part, 3, 1);
+    put<RSUM>(bad, red, part, 3, 2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_clamp_sum(std::int64_t N, T* __restrict__ p, double* __restrict__ part) {
+    __shared__ double red[8];
+    double s = 0.0;
+    GRID_LOOP(i, N) {
+        const T v = p[i];
+        const T c = (v < T(0)) ? T(0) : v;   // cwiseMax(0)
+        p[i] = c;
+        s += (double)c;
+    }
+    put<RSUM>(s, red, part, 1, 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scale(std::int64_t N, T* __restrict__ p, T f) {
+    GRID_LOOP(i, N) p[i] = p[i] * f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(std::int64_t N, T* __restrict__ p, T v) {
+    GRID_LOOP(i, N) p[i] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_from_double(std::int64_t N, const double* __restrict__ x, double shift, T* __restrict__ y) {
+    GRID_LOOP(i, N) y[i] = static_cast<T>(x[i] - shift);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_masked_sum(std::int64_t N, const T* __restrict__ y, const std::uint8_t* __restrict__ mask,
+             double* __restrict__ part) {
+    __shared__ double red[8];
+    double s = 0.0, c = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        s += (double)y[i];
+        c += 1.0;
+    }
+    put<RSUM>(s, red, part, 2, 0);
+    put<RSUM>(c, red, part, 2, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_masked_sub(std::int64_t N, T* __restrict__ y, const std::uint8_t* __restrict__ mask,
+             const double* __restrict__ scal, int idx) {
+    const T c = static_cast<T>(scal[idx]);
+    GRID_LOOP(i, N) if (mask[i]) y[i] = y[i] - c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_shift(std::int64_t N, const T* __restrict__ x, T c, T* __restrict__ y) {
+    GRID_LOOP(i, N) y[i] = x[i] - c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_to_double(std::int64_t N, const T* __restrict__ x, double* __restrict__ y) {
+    GRID_LOOP(i, N) y[i] = (double)x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dense(int Ns, const double* __restrict__ table, const T* __restrict__ x, T* __restrict__ y) {
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    GRID_LOOP(i, N) {
+        const int iy = (int)(i / Ns), ix = (int)(i - (std::int64_t)iy * Ns);
+        double acc = 0.0;
+        for (int jy = 0; jy < Ns; ++jy) {
+            const double* trow = table + (std::size_t)abs(iy - jy) * Ns;
+            const T* xr = x + (std::size_t)jy * Ns;
+            for (int jx = 0; jx < Ns; ++jx) acc += trow[abs(ix - jx)] * (double)xr[jx];
+        }
+        y[i] = static_cast<T>(acc);
+    }
+}
+
+// ── finalize: partials -> scalars, then the derived-scalar step ────────────
+
+template <typename T>
+__device__ void run_step(int step, double* s) {
+    const double nc = s[S_NC];
+    const double P = s[S_PTOT], gref = s[S_GREF];
+    switch (step) {
+        case STEP_EVAL1: {
+            const double alpha = nc > 0.0 ? s[S_GSUM] / nc : s[S_VMIN];
+            s[S_ALPHA] = alpha;
+            s[S_GMIN] = s[S_VMIN] - alpha;
+            break;
+        }
+        case STEP_EVAL2: {
+            double a = s[S_ALPHA], gm = s[S_GMIN];
+            if (nc > 0.0) {  // refine_contact_mean (src/contact_solver.cpp:94-99)
+                const double delta = s[S_GCS] / nc;
+                a += delta;
+                gm -= delta;
+            }
+            s[S_ALPHA_REF] = a;
+            s[S_GMIN_REF] = gm;
+            s[S_PK] = s[S_E] / (P * gref);
+            s[S_FW] = s[S_GC] / (P * gref);
+            s[S_PEN] = ((-gm > 0.0) ? -gm : 0.0) / gref;
+            s[S_LOAD] = fabs(s[S_PSUM] - P) / P;
+            s[S_FINITE] = (isfinite(s[S_GSUM]) && isfinite(s[S_VMIN]) &&
+                           isfinite(s[S_PSUM]) && isfinite(s[S_E]) && isfinite(s[S_GC]))
+                              ? 1.0 : 0.0;
+            s[S_GMEAN] = nc > 0.0 ? s[S_GCS] / nc : 0.0;
+            break;
+        }
+        case STEP_ZMEAN:
+            s[S_ZMEAN] = nc > 0.0 ? s[S_ZSUM] / nc : 0.0;
+            break;
+        case STEP_CGDOT: {
+            const double Gold = s[S_GOLD];
+            double bv = 0.0;
+            if (Gold != 0.0) {
+                if (s[S_USEPR] != 0.0) {
+                    const double q = s[S_GPR] / Gold;
+                    bv = (0.0 < q) ? q : 0.0;   // std::max(0.0, q)
+                } else {
+                    bv = s[S_G] / Gold;
+                }
+            }
+            s[S_BETA] = (double)(static_cast<T>(s[S_DELTA]) * static_cast<T>(bv));
+            s[S_GOLD] = s[S_G];
+            break;
+        }
+        case STEP_LINE1:
+            s[S_RMEAN] = (double)static_cast<T>(nc > 0.0 ? s[S_RSUM] / nc : 0.0);
+            break;
+        case STEP_LINE2: {
+            const double den = s[S_DEN], num = s[S_NUM];
+            const bool ok = den > 0.0 && isfinite(den) && isfinite(num);
+            s[S_OK] = ok ? 1.0 : 0.0;
+            if (ok) {
+                const T tau = static_cast<T>(num / den);
+                const double gg = s[S_GG];
+                s[S_TAU] = (double)tau;
+                s[S_TAUOV] = gg > 0.0 ? (double)static_cast<T>(num * num / (den * gg))
+                                      : (double)tau;
+            }
+            break;
+        }
+        case STEP_UPDATE:
+            if (s[S_OK] != 0.0) {
+                s[S_DELTA] = s[S_CHANGED] != 0.0 ? 0.0 : 1.0;
+                const double total = s[S_TOTAL];
+                if (total > 0.0) {
+                    s[S_F] = (double)static_cast<T>(P / total);
+                    s[S_PFILL] = 0.0;
+                } else {
+                    s[S_PFILL] = 1.0;
+                }
+            }
+            break;
+        default:
+            break;
+    }
+}
+
+struct FinSpec {
+    int nv;
+    int ops[kPartVals];
+    int dst[kPartVals];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __restrict__ scal,
+           int step) {
+    __shared__ double red[8];
+    for (int v = 0; v < spec.nv; ++v) {
+        const int op = spec.ops[v];
+        double acc = op == RSUM ? 0.0 : (op == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+            const double x = part[(std::size_t)b * kPartVals + v];
+            acc = op == RSUM ? acc + x : (op == RMIN ? ((x < acc) ? x : acc) : ((x > acc) ? x : acc));
+        }
+        double r;
+        if (op == RSUM) r = block_reduce<RSUM>(acc, red);
+        else if (op == RMIN) r = block_reduce<RMIN>(acc, red);
+        else r = block_reduce<RMAX>(acc, red);
+        if (threadIdx.x == 0) scal[spec.dst[v]] = r;
+    }
+    if (threadIdx.x == 0) run_step<T>(step, scal);
+}
+
+struct SetSpec {
+    int n;
+    int idx[8];
+    double val[8];
+};
+
+__global__ void k_set_scal(double* scal, SetSpec s) {
+    if (threadIdx.x < s.n) scal[s.idx[threadIdx.x]] = s.val[threadIdx.x];
+}
+
+// ── Love kernel (src/boussinesq_kernel.cpp:44-95) ──────────────────────────
+
+__device__ double d_asinh_diff(double u, double v, double du) {
+    const double huge_arg = 1e150;
+    if (fabs(u) > huge_arg || fabs(v) > huge_arg) return asinh(u) - asinh(v);
+    const double su = sqrt(1.0 + u * u), sv = sqrt(1.0 + v * v);
+    const double den = u * sv + v * su;
+    const double w = (u * v > 0.0 && den != 0.0) ? du * (u + v) / den : u * sv - v * su;
+    return asinh(w);
+}
+
+__device__ double d_term(double t, double s1, double s2, double ds) {
+    if (t == 0.0) return 0.0;
+    const double c = fabs(t);
+    return t * d_asinh_diff(s1 / c, s2 / c, ds / c);
+}
+
+__device__ double d_love_uz(double x, double y, double a, double b) {
+    x = fabs(x);
+    y = fabs(y);
+    const double m = (a > b) ? a : b;
+    const double r2 = x * x + y * y;
+    const double sw = 1500.0 * m;
+    if (r2 > sw * sw) {
+        const double r = sqrt(r2);
+        const double A = 4.0 * a * b;
+        const double quad = (a * a * (3.0 * x * x - r2) + b * b * (3.0 * y * y - r2)) /
+                            (6.0 * r2 * r2);
+        return A * (1.0 + quad) / r;
+    }
+    const double xp = x + a, xm = x - a, yp = y + b, ym = y - b;
+    const double da = 2.0 * a, db = 2.0 * b;
+    return d_term(xp, yp, ym, db) + d_term(yp, xp, xm, da) + d_term(xm, ym, yp, -db) +
+           d_term(ym, xm, xp, -da);
+}
+
+__global__ void __launch_bounds__(256)
+k_love_table(int Ns, double h, double scale, double* __restrict__ table) {
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    const double a = 0.5 * h;
+    GRID_LOOP(i, N) {
+        const int dy = (int)(i / Ns), dx = (int)(i - (std::int64_t)dy * Ns);
+        table[i] = scale * d_love_uz(dx * h, dy * h, a, a);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_love_points(int n, const double* x, const double* y, const double* a, const double* b,
+              double* out) {
+    GRID_LOOP(i, n) out[i] = d_love_uz(x[i], y[i], a[i], b[i]);
+}
+
+__global__ void __launch_bounds__(256)
+k_restrict2(int Ns, const double* __restrict__ f, double* __restrict__ out) {
+    const int c = Ns / 2;
+    const std::int64_t N = (std::int64_t)c * c;
+    GRID_LOOP(i, N) {
+        const int iy = (int)(i / c), ix = (int)(i - (std::int64_t)iy * c);
+        const std::size_t x0 = 2 * ix, y0 = 2 * iy;
+        out[i] = 0.25 * (f[y0 * Ns + x0] + f[y0 * Ns + x0 + 1] + f[(y0 + 1) * Ns + x0] +
+                         f[(y0 + 1) * Ns + x0 + 1]);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_prolong2(int Nc, const double* __restrict__ pc, double* __restrict__ out) {
+    const int f = 2 * Nc;
+    const std::int64_t N = (std::int64_t)f * f;
+    GRID_LOOP(i, N) {
+        const int iy = (int)(i / f), ix = (int)(i - (std::int64_t)iy * f);
+        out[i] = pc[(std::size_t)(iy >> 1) * Nc + (ix >> 1)];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_cast_d2f(std::int64_t N, const double* __restrict__ x, float* __restrict__ y) {
+    GRID_LOOP(i, N) y[i] = (float)x[i];
+}
+
+} // namespace
+
+#define L1(k, ...) k<<<nblk, kVecThreads, 0, s>>>(__VA_ARGS__); ASPHER_LAUNCH_CHECK()
+#define L1S(k, ...) k<<<vec_blocks(N), kVecThreads, 0, s>>>(__VA_ARGS__); ASPHER_LAUNCH_CHECK()
+
+template <typename T>
+void VecOps<T>::eval1(std::int64_t N, const T* u, const T* g0s, const T* p, T* g,
+                      double* part, int nblk, hipStream_t s) {
+    L1(k_eval1<T>, N, u, g0s, p, g, part);
+}
+template <typename T>
+void VecOps<T>::eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
+                      const double* scal, double* part, int nblk, hipStream_t s) {
+    L1(k_eval2<T>, N, p, g, mask, scal, part);
+}
+template <typename T>
+void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const double* scal,
+                        const int* tapd, const double* tapw, int ntaps, T* z,
+                        double* part, int nblk, hipStream_t s) {
+    L1(k_stencil<T>, Ns, g, mask, scal, tapd, tapw, ntaps, z, part);
+}
+template <typename T>
+void VecOps<T>::cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
+                      const std::uint8_t* mask, const double* scal, bool has_pc,
+                      double* part, int nblk, hipStream_t s) {
+    if (has_pc) { L1((k_cgdot<T, true>), N, g, gprev, z, mask, scal, part); }
+    else { L1((k_cgdot<T, false>), N, g, gprev, z, mask, scal, part); }
+}
+template <typename T>
+void VecOps<T>::tupdate(std::int64_t N, const T* g, T* gprev, const T* z,
+                        const std::uint8_t* mask, T* t, const double* scal, bool has_pc,
+                        hipStream_t s) {
+    if (has_pc) { L1S((k_tupdate<T, true>), N, g, gprev, z, mask, t, scal); }
+    else { L1S((k_tupdate<T, false>), N, g, gprev, z, mask, t, scal); }
+}
+template <typename T>
+void VecOps<T>::line1(std::int64_t N, const T* r, const T* g, const T* t,
+                      const std::uint8_t* mask, double* part, int nblk, hipStream_t s) {
+    L1(k_line1<T>, N, r, g, t, mask, part);
+}
+template <typename T>
+void VecOps<T>::line2(std::int64_t N, const T* r, const T* t, const std::uint8_t* mask,
+                      const double* scal, double* part, int nblk, hipStream_t s) {
+    L1(k_line2<T>, N, r, t, mask, scal, part);
+}
+template <typename T>
+void VecOps<T>::update(std::int64_t N, T* p, const T* t, const T* g, const double* scal,
+                       double* part, int nblk, hipStream_t s) {
+    L1(k_update<T>, N, p, t, g, scal, part);
+}
+template <typename T>
+void VecOps<T>::rescale(std::int64_t N, T* p, const double* scal, hipStream_t s) {
+    L1S(k_rescale<T>, N, p, scal);
+}
+template <typename T>
+void VecOps<T>::objective(std::int64_t N, const T* p, const T* u, const T* g0s,
+                          double* part, int nblk, hipStream_t s) {
+    L1(k_objective<T>, N, p, u, g0s, part);
+}
+template <typename T>
+void VecOps<T>::axpy_id(std::int64_t N, const T* p, const T* g, T* z, double s_id,
+                        hipStream_t s) {
+    L1S(k_axpy_id<T>, N, p, g, z, static_cast<T>(s_id));
+}
+template <typename T>
+void VecOps<T>::sum(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s) {
+    L1(k_sum<T>, N, y, part);
+}
+template <typename T>
+void VecOps<T>::count_above(std::int64_t N, const T* y, double theta, double* part,
+                            int nblk, hipStream_t s) {
+    L1(k_count_above<T>, N, y, theta, part);
+}
+template <typename T>
+void VecOps<T>::project(std::int64_t N, T* y, double theta, hipStream_t s) {
+    L1S(k_project<T>, N, y, static_cast<T>(theta));
+}
+template <typename T>
+void VecOps<T>::idcheck(std::int64_t N, const T* z, const T* r, const T* g0s, const T* g,
+                        const T* p, double* part, int nblk, hipStream_t s) {
+    L1(k_idcheck<T>, N, z, r, g0s, g, p, part);
+}
+template <typename T>
+void VecOps<T>::stats(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s) {
+    L1(k_stats<T>, N, y, part);
+}
+template <typename T>
+void VecOps<T>::clamp_sum(std::int64_t N, T* p, double* part, int nblk, hipStream_t s) {
+    L1(k_clamp_sum<T>, N, p, part);
+}
+template <typename T>
+void VecOps<T>::scale(std::int64_t N, T* p, double f, hipStream_t s) {
+    L1S(k_scale<T>, N, p, static_cast<T>(f));
+}
+template <typename T>
+void VecOps<T>::fill(std::int64_t N, T* p, double v, hipStream_t s) {
+    L1S(k_fill<T>, N, p, static_cast<T>(v));
+}
+template <typename T>
+void VecOps<T>::from_double(std::int64_t N, const double* x, double shift, T* y,
+                            hipStream_t s) {
+    L1S(k_from_double<T>, N, x, shift, y);
+}
+template <typename T>
+void VecOps<T>::to_double(std::int64_t N, const T* x, double* y, hipStream_t s) {
+    L1S(k_to_double<T>, N, x, y);
+}
+template <typename T>
+void VecOps<T>::shift(std::int64_t N, const T* x, double c, T* y, hipStream_t s) {
+    L1S(k_shift<T>, N, x, static_cast<T>(c), y);
+}
+template <typename T>
+void VecOps<T>::masked_sum(std::int64_t N, const T* y, const std::uint8_t* mask, double* part,
+                           int nblk, hipStream_t s) {
+    L1(k_masked_sum<T>, N, y, mask, part);
+}
+template <typename T>
+void VecOps<T>::masked_sub(std::int64_t N, T* y, const std::uint8_t* mask, const double* scal,
+                           int idx, hipStream_t s) {
+    L1S(k_masked_sub<T>, N, y, mask, scal, idx);
+}
+template <typename T>
+void VecOps<T>::copy(std::int64_t N, const T* x, T* y, hipStream_t s) {
+    ASPHER_HIP(hipMemcpyAsync(y, x, sizeof(T) * N, hipMemcpyDeviceToDevice, s));
+}
+template <typename T>
+void VecOps<T>::dense(int Ns, const double* table, const T* x, T* y, hipStream_t s) {
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    k_dense<T><<<(int)((N + 255) / 256), 256, 0, s>>>(Ns, table, x, y);
+    ASPHER_LAUNCH_CHECK();
+}
+
+template struct VecOps<float>;
+template struct VecOps<double>;
+
+template <typename T>
+void finalize(const double* part, int nblk, int nv, const int* ops, const int* dst,
+              double* scal, Step step, hipStream_t s) {
+    FinSpec f{};
+    f.nv = nv;
+    for (int v = 0; v < nv; ++v) {
+        f.ops[v] = ops[v];
+        f.dst[v] = dst[v];
+    }
+    k_finalize<T><<<1, 256, 0, s>>>(part, nblk, f, scal, static_cast<int>(step));
+    ASPHER_LAUNCH_CHECK();
+}
+template void finalize<float>(const double*, int, int, const int*, const int*, double*,
+                              Step, hipStream_t);
+template void finalize<double>(const double*, int, int, const int*, const int*, double*,
+                               Step, hipStream_t);
+
+void set_scal(double* scal, int n, const int* idx, const double* val, hipStream_t s) {
+    SetSpec sp{};
+    if (n > 8) throw std::invalid_argument("set_scal: at most 8 values");
+    sp.n = n;
+    for (int i = 0; i < n; ++i) {
+        sp.idx[i] = idx[i];
+        sp.val[i] = val[i];
+    }
+    k_set_scal<<<1, 64, 0, s>>>(scal, sp);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void love_table(int Ns, double L, double E_star, double* table, hipStream_t s) {
+    const double h = L / Ns;
+    const double scale = 1.0 / (M_PI * E_star);
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    k_love_table<<<vec_blocks(N), kVecThreads, 0, s>>>(Ns, h, scale, table);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void love_points(int n, const double* x, const double* y, const double* a, const double* b,
+                 double* out, hipStream_t s) {
+    k_love_points<<<vec_blocks(n), kVecThreads, 0, s>>>(n, x, y, a, b, out);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void restrict2(int Ns, const double* f, double* out, hipStream_t s) {
+    const std::int64_t N = (std::int64_t)(Ns / 2) * (Ns / 2);
+    k_restrict2<<<vec_blocks(N), kVecThreads, 0, s>>>(Ns, f, out);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void prolong2(int Nc, const double* pc, double* out, hipStream_t s) {
+    const std::int64_t N = 4LL * Nc * Nc;
+    k_prolong2<<<vec_blocks(N), kVecThreads, 0, s>>>(Nc, pc, out);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void cast_d2f(std::int64_t N, const double* x, float* y, hipStream_t s) {
+    k_cast_d2f<<<vec_blocks(N), kVecThreads, 0, s>>>(N, x, y);
+    ASPHER_LAUNCH_CHECK();
+}
+
+} // namespace aspher

@@ -1,0 +1,120 @@
+// Host launch wrappers for the O(N) kernels of the projected CG, the Love
+// table, the dense operator and the nested-grid transfers. All pointers are
+// device pointers; every launch is asynchronous on the given stream.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstdint>
+
+namespace aspher {
+
+// Device scalar block: indices into a double array shared by the kernels of
+// one solve. Derived scalars are computed on the device by the finalize
+// steps, so the iteration body needs one host read per iteration.
+enum Scal : int {
+    S_VMIN, S_GSUM, S_NC, S_ALPHA, S_GMIN,
+    S_PV, S_GCS, S_E, S_GC, S_PSUM,
+    S_PK, S_FW, S_PEN, S_LOAD, S_FINITE, S_ALPHA_REF, S_GMIN_REF, S_GMEAN,
+    S_G, S_GPR, S_GG, S_BETA, S_GOLD, S_DELTA,
+    S_RSUM, S_NUM, S_RMEAN, S_DEN, S_OK, S_TAU, S_TAUOV,
+    S_CHANGED, S_TOTAL, S_F, S_PFILL,
+    S_ZSUM, S_ZMEAN,
+    S_PTOT, S_GREF, S_USEPR, S_PBAR,
+    S_R0, S_R1, S_R2, S_R3, S_R4, S_R5,   // generic reduction results
+    S_COUNT
+};
+
+enum Step : int {
+    STEP_NONE = 0, STEP_EVAL1, STEP_EVAL2, STEP_ZMEAN, STEP_CGDOT, STEP_LINE1,
+    STEP_LINE2, STEP_UPDATE,
+};
+
+template <typename T> struct VecOps {
+    // evaluate pass 1: g = u + g0s; partials {sum_{p>0} v, n_{p>0}, min v}
+    static void eval1(std::int64_t N, const T* u, const T* g0s, const T* p, T* g,
+                      double* part, int nblk, hipStream_t s);
+    // evaluate pass 2 (uses S_ALPHA, S_GMIN): partials {pv, gcs, e, Gc, psum};
+    // writes mask = p > 0
+    static void eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
+                      const double* scal, double* part, int nblk, hipStream_t s);
+    // 13-tap (any radius) masked stencil (uses S_GMEAN): z and partial zsum
+    static void stencil(int Ns, const T* g, const std::uint8_t* mask,
+                        const double* scal, const int* tapd, const double* tapw,
+                        int ntaps, T* z, double* part, int nblk, hipStream_t s);
+    // {G, G_pr, gg} over the mask; z is (z - zmean) when has_pc, else g
+    static void cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
+                      const std::uint8_t* mask, const double* scal, bool has_pc,
+                      double* part, int nblk, hipStream_t s);
+    // t = mask ? z + beta t : 0 ; gprev = g
+    static void tupdate(std::int64_t N, const T* g, T* gprev, const T* z,
+                        const std::uint8_t* mask, T* t, const double* scal,
+                        bool has_pc, hipStream_t s);
+    static void line1(std::int64_t N, const T* r, const T* g, const T* t,
+                      const std::uint8_t* mask, double* part, int nblk, hipStream_t s);
+    static void line2(std::int64_t N, const T* r, const T* t, const std::uint8_t* mask,
+                      const double* scal, double* part, int nblk, hipStream_t s);
+    // projected update + overlap step (predicated on S_OK): partials {changed, total}
+    static void update(std::int64_t N, T* p, const T* t, const T* g,
+                       const double* scal, double* part, int nblk, hipStream_t s);
+    // load rescale (predicated on S_OK)
+    static void rescale(std::int64_t N, T* p, const double* scal, hipStream_t s);
+    // {sum p(0.5u+g0s), sum p^2, sum p u}
+    static void objective(std::int64_t N, const T* p, const T* u, const T* g0s,
+                          double* part, int nblk, hipStream_t s);
+    // z = p - s g
+    static void axpy_id(std::int64_t N, const T* p, const T* g, T* z, double s_id,
+                        hipStream_t st);
+    static void sum(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s);
+    // {sum_{y>theta} y, count_{y>theta}}
+    static void count_above(std::int64_t N, const T* y, double theta, double* part,
+                            int nblk, hipStream_t s);
+    static void project(std::int64_t N, T* y, double theta, hipStream_t s);
+    // {sum z(0.5 r + g0s), sum g (z - p)}
+    static void idcheck(std::int64_t N, const T* z, const T* r, const T* g0s,
+                        const T* g, const T* p, double* part, int nblk, hipStream_t s);
+    // {min, max, non-finite count} of a field
+    static void stats(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s);
+    // p = max(p, 0); partial {sum p}
+    static void clamp_sum(std::int64_t N, T* p, double* part, int nblk, hipStream_t s);
+    static void scale(std::int64_t N, T* p, double f, hipStream_t s);
+    static void fill(std::int64_t N, T* p, double v, hipStream_t s);
+    // y = T(x - shift) from double
+    static void from_double(std::int64_t N, const double* x, double shift, T* y,
+                            hipStream_t s);
+    static void to_double(std::int64_t N, const T* x, double* y, hipStream_t s);
+    // y = x - T(c), in working precision (datum removal)
+    static void shift(std::int64_t N, const T* x, double c, T* y, hipStream_t s);
+    static void copy(std::int64_t N, const T* x, T* y, hipStream_t s);
+    // {sum_mask y, count_mask}
+    static void masked_sum(std::int64_t N, const T* y, const std::uint8_t* mask,
+                           double* part, int nblk, hipStream_t s);
+    // y = mask ? y - T(scal[idx]) : y
+    static void masked_sub(std::int64_t N, T* y, const std::uint8_t* mask,
+                           const double* scal, int idx, hipStream_t s);
+    static void dense(int Ns, const double* table, const T* x, T* y, hipStream_t s);
+};
+
+// Reduce `nv` partial columns (ops: RSUM/RMIN/RMAX) into scal[dst[v]] and run
+// a finalize step. T selects the working-precision casts of the step.
+template <typename T>
+void finalize(const double* part, int nblk, int nv, const int* ops, const int* dst,
+              double* scal, Step step, hipStream_t s);
+
+// Set a few scalars from the host (a kernel, so the stream order holds).
+void set_scal(double* scal, int n, const int* idx, const double* val, hipStream_t s);
+
+// Love table T[dy*Ns+dx] = love_uz(dx h, dy h, h/2, h/2)/(pi E*), FP64.
+void love_table(int Ns, double L, double E_star, double* table, hipStream_t s);
+// love_uz at arbitrary points (tests of the device kernel)
+void love_points(int n, const double* x, const double* y, const double* a,
+                 const double* b, double* out, hipStream_t s);
+
+// 2x2 block-average restriction (Ns -> Ns/2) and injection prolongation.
+void restrict2(int Ns, const double* f, double* out, hipStream_t s);
+void prolong2(int Nc, const double* pc, double* out, hipStream_t s);
+
+// float <-> double spectrum copy
+void cast_d2f(std::int64_t N, const double* x, float* y, hipStream_t s);
+
+} // namespace aspher

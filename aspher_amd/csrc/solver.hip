@@ -1,0 +1,711 @@
+// Certified Polonsky–Keer projected CG (src/contact_solver.cpp:150-587) and
+// the nested-grid driver for the fft / periodic backends
+// (src/nested_solve.cpp:518-990), on device-resident state.
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "reduce.hpp"
+#include "timer.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
+
+namespace aspher {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+inline double since(Clock::time_point t0) {
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+const int kS1[1] = {RSUM};
+const int kS2[2] = {RSUM, RSUM};
+const int kS3[3] = {RSUM, RSUM, RSUM};
+const int kS5[5] = {RSUM, RSUM, RSUM, RSUM, RSUM};
+
+struct Diag {
+    double alpha = 0, gmin = 0, pk = 0, G = 0, fw = 0, pen = 0, load = 0, psum = 0, pv = 0;
+    long long nc = 0;
+    bool finite = true;
+};
+
+Diag read_diag(const double* h) {
+    Diag d;
+    d.alpha = h[S_ALPHA_REF];
+    d.gmin = h[S_GMIN_REF];
+    d.pk = h[S_PK];
+    d.G = h[S_GC];
+    d.fw = h[S_FW];
+    d.pen = h[S_PEN];
+    d.load = h[S_LOAD];
+    d.psum = h[S_PSUM];
+    d.pv = h[S_PV];
+    d.nc = (long long)h[S_NC];
+    d.finite = h[S_FINITE] != 0.0;
+    return d;
+}
+
+} // namespace
+
+template <typename Real>
+void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
+                const SolveOptions& opt, Precond* pc, Real* p_init, Result& res) {
+    using V = VecOps<Real>;
+    constexpr bool is_double = std::is_same_v<Real, double>;
+    hipStream_t s = ctx.stream;
+    const int Ns = op.Ns();
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    if (N == 0 || !(p_bar > Real(0)))
+        throw std::invalid_argument("solve_contact: empty gap or p_bar <= 0");
+    if (!(opt.tol > 0.0)) throw std::invalid_argument("solve_contact: tol must be positive");
+    if (opt.max_iter < 0) throw std::invalid_argument("solve_contact: max_iter must be >= 0");
+    const auto t_start = Clock::now();
+    const int nblk = vec_blocks(N);
+    double* scal = ctx.scal.p;
+    double* part = ctx.part.p;
+    const double* hs = ctx.host_scal;
+
+    // non-finite input is rejected before any operator call
+    {
+        static const int ops[3] = {RMIN, RMAX, RSUM};
+        static const int dst[3] = {S_R0, S_R1, S_R2};
+        V::stats(N, g0, part, nblk, s);
+        finalize<Real>(part, nblk, 3, ops, dst, scal, STEP_NONE, s);
+        ctx.read_scal();
+    }
+    if (hs[S_R2] != 0.0)
+        throw std::invalid_argument("solve_contact: g0 contains non-finite values");
+    const double gmin_v = hs[S_R0], gmax_v = hs[S_R1];
+
+    const double P_total = static_cast<double>(p_bar) * N;
+    double g_ref = opt.g_ref;
+    if (!(g_ref > 0.0)) g_ref = gmax_v - gmin_v;
+    if (!(g_ref > 0.0)) g_ref = 1.0;
+    const double p_ref = opt.p_ref > 0.0 ? opt.p_ref : static_cast<double>(p_bar);
+    const double load_tol = opt.load_tol > 0.0 ? opt.load_tol : (is_double ? 1e-12 : 5e-7);
+    const double tol = opt.tol;
+    const double req_tol = opt.requested_tol > 0.0 ? opt.requested_tol : tol;
+    double datum = 0.0;
+    bool subtract = false;
+    switch (opt.datum_mode) {
+        case SolveOptions::Datum::automatic:
+            datum = gmin_v + 0.5 * (gmax_v - gmin_v);
+            subtract = true;
+            break;
+        case SolveOptions::Datum::solver:
+            datum = opt.datum;
+            subtract = true;
+            break;
+        case SolveOptions::Datum::caller:
+            datum = opt.datum;
+            break;
+    }
+    const Real gshift = subtract ? static_cast<Real>(datum) : Real(0);
+
+    // device state: p, t, u, g, g_prev, z, r (+ best p, mask, centred gap)
+    DevBuf<Real> bg0s(N), bp(N), bt(N), bu(N), bg(N), bgp(N), bz(N), br(N), bbest;
+    DevBuf<std::uint8_t> bmask(N);
+    Real *P = bp.p, *T_ = bt.p, *U = bu.p, *G = bg.p, *GP = bgp.p, *Z = bz.p, *R = br.p;
+    Real* g0s = bg0s.p;
+    std::uint8_t* mask = bmask.p;
+    V::shift(N, g0, static_cast<double>(gshift), g0s, s);
+
+    if (p_init) {
+        V::copy(N, p_init, P, s);
+        V::clamp_sum(N, P, part, nblk, s);
+        static const int d1[1] = {S_R0};
+        finalize<Real>(part, nblk, 1, kS1, d1, scal, STEP_NONE, s);
+        ctx.read_scal();
+        const double ssum = hs[S_R0];
+        if (ssum > 0.0) V::scale(N, P, static_cast<double>(static_cast<Real>(P_total / ssum)), s);
+        else V::fill(N, P, static_cast<double>(p_bar), s);
+    } else {
+        V::fill(N, P, static_cast<double>(p_bar), s);
+    }
+    V::fill(N, T_, 0.0, s);
+    V::fill(N, GP, 0.0, s);
+    {
+        const int idx[6] = {S_PTOT, S_GREF, S_USEPR, S_PBAR, S_GOLD, S_DELTA};
+        const double val[6] = {P_total, g_ref, opt.use_pr ? 1.0 : 0.0,
+                               static_cast<double>(p_bar), 1.0, 0.0};
+        set_scal(scal, 6, idx, val, s);
+    }
+    if (opt.keep_best) bbest.alloc(N);
+
+    long long mv = 0, pcount = 0;
+    EventTimer tm_mv, tm_pc;
+    auto sync_read = [&]() {
+        ctx.read_scal();
+        tm_mv.drain();
+        tm_pc.drain();
+    };
+    auto apply = [&](const Real* x, Real* y) {
+        tm_mv.start(s);
+        op.apply(x, y, s);
+        tm_mv.stop(s);
+        ++mv;
+    };
+    auto evaluate = [&]() {
+        static const int o1[3] = {RSUM, RSUM, RMIN};
+        static const int d1[3] = {S_GSUM, S_NC, S_VMIN};
+        static const int d2[5] = {S_PV, S_GCS, S_E, S_GC, S_PSUM};
+        V::eval1(N, U, g0s, P, G, part, nblk, s);
+        finalize<Real>(part, nblk, 3, o1, d1, scal, STEP_EVAL1, s);
+        V::eval2(N, P, G, mask, scal, part, nblk, s);
+        finalize<Real>(part, nblk, 5, kS5, d2, scal, STEP_EVAL2, s);
+    };
+    auto set_delta_restart = [&]() {
+        const int idx[2] = {S_DELTA, S_GOLD};
+        const double val[2] = {0.0, 1.0};
+        set_scal(scal, 2, idx, val, s);
+    };
+
+    // simplex projection (src/contact_solver.cpp:31-60), host-driven passes
+    auto project_load_simplex = [&](Real* y) {
+        static const int d1[1] = {S_R0};
+        static const int d2[2] = {S_R0, S_R1};
+        V::sum(N, y, part, nblk, s);
+        finalize<Real>(part, nblk, 1, kS1, d1, scal, STEP_NONE, s);
+        ctx.read_scal();
+        double ssum = hs[S_R0];
+        double nact = (double)N;
+        double theta = (ssum - P_total) / nact;
+        for (int pass = 0; pass < 64; ++pass) {
+            V::count_above(N, y, theta, part, nblk, s);
+            finalize<Real>(part, nblk, 2, kS2, d2, scal, STEP_NONE, s);
+            ctx.read_scal();
+            const double s2 = hs[S_R0], n2 = hs[S_R1];
+            if (n2 == 0.0) break;
+            if (n2 == nact) break;
+            nact = n2;
+            ssum = s2;
+            theta = (ssum - P_total) / nact;
+        }
+        V::project(N, y, theta, s);
+    };
+
+    // feasible identification step (src/contact_solver.cpp:314-364)
+    double s_id = 0.0;
+    bool u_fresh = false;
+    auto identification_step = [&]() -> bool {
+        static const int d3[3] = {S_R0, S_R1, S_R2};
+        static const int d2[2] = {S_R0, S_R1};
+        V::objective(N, P, U, g0s, part, nblk, s);
+        finalize<Real>(part, nblk, 3, kS3, d3, scal, STEP_NONE, s);
+        sync_read();
+        const double f_cur = hs[S_R0], pp = hs[S_R1], pu = hs[S_R2];
+        if (!(s_id > 0.0)) s_id = (pu > 0.0 && pp > 0.0) ? pp / pu : 1.0;
+        for (int trial = 0; trial < 12; ++trial) {
+            V::axpy_id(N, P, G, Z, s_id, s);
+            project_load_simplex(Z);
+            apply(Z, R);
+            V::idcheck(N, Z, R, g0s, G, P, part, nblk, s);
+            finalize<Real>(part, nblk, 2, kS2, d2, scal, STEP_NONE, s);
+            sync_read();
+            const double f_try = hs[S_R0], gd = hs[S_R1];
+            if (std::isfinite(f_try) && gd < 0.0 && f_try <= f_cur + 1e-4 * gd) {
+                std::swap(P, Z);
+                std::swap(U, R);
+                u_fresh = true;
+                set_delta_restart();
+                V::fill(N, T_, 0.0, s);
+                V::fill(N, GP, 0.0, s);
+                ++res.identification_steps;
+                if (trial == 0) s_id *= 2.0;
+                return true;
+            }
+            s_id *= 0.25;
+        }
+        return false;
+    };
+
+    bool precond_on = pc && pc->kind() != Precond::Kind::none;
+    bool precond_dropped = false;
+
+    // one CG step: direction, second operator apply, line search, projected
+    // update and load rescale; the update is predicated on the device
+    auto launch_cg = [&]() {
+        static const int dg[3] = {S_G, S_GPR, S_GG};
+        static const int dl1[2] = {S_RSUM, S_NUM};
+        static const int dl2[1] = {S_DEN};
+        static const int ou[2] = {RMAX, RSUM};
+        static const int du[2] = {S_CHANGED, S_TOTAL};
+        const bool has_pc = precond_on;
+        if (has_pc) {
+            tm_pc.start(s);
+            pc->apply<Real>(G, mask, Z, scal, part, s);
+            tm_pc.stop(s);
+            ++pcount;
+            if (opt.precond_cost_gate > 0.0 && pcount >= 1 && mv >= 1) {
+                // opt-in cost gate (src/contact_solver.cpp:132-146)
+                sync_read();
+                const double per_pc = tm_pc.seconds() / pcount;
+                const double per_mv = tm_mv.seconds() / mv;
+                if (per_mv > 0.0 && per_pc > opt.precond_cost_gate * per_mv) {
+                    precond_on = false;
+                    precond_dropped = true;
+                    const int idx[1] = {S_DELTA};
+                    const double val[1] = {0.0};
+                    set_scal(scal, 1, idx, val, s);
+                }
+            }
+        }
+        V::cgdot(N, G, GP, Z, mask, scal, has_pc, part, nblk, s);
+        finalize<Real>(part, nblk, 3, kS3, dg, scal, STEP_CGDOT, s);
+        V::tupdate(N, G, GP, Z, mask, T_, scal, has_pc, s);
+        apply(T_, R);
+        V::line1(N, R, G, T_, mask, part, nblk, s);
+        finalize<Real>(part, nblk, 2, kS2, dl1, scal, STEP_LINE1, s);
+        V::line2(N, R, T_, mask, scal, part, nblk, s);
+        finalize<Real>(part, nblk, 1, kS1, dl2, scal, STEP_LINE2, s);
+        V::update(N, P, T_, G, scal, part, nblk, s);
+        finalize<Real>(part, nblk, 2, ou, du, scal, STEP_UPDATE, s);
+        V::rescale(N, P, scal, s);
+    };
+
+    Status status = Status::max_iterations;
+    std::string reason;
+    double best_merit = std::numeric_limits<double>::infinity();
+    double seen_merit = std::numeric_limits<double>::infinity();
+    int stall = 0, forced_id = 0;
+    const int stall_limit = opt.stall_limit > 0 ? opt.stall_limit : 200;
+    bool have_best = false;
+    bool pending = false;
+    Diag d;
+
+    int it = 0;
+    for (it = 0; it < opt.max_iter; ++it) {
+        if (!u_fresh) apply(P, U);
+        u_fresh = false;
+        evaluate();
+        sync_read();
+        d = read_diag(hs);
+        if (pending) {
+            pending = false;
+            if (hs[S_OK] == 0.0) {
+                // the previous step had no usable curvature: run its
+                // identification step now, on the identical state
+                if (!identification_step()) {
+                    status = Status::nonpositive_curvature;
+                    --it;
+                    break;
+                }
+                u_fresh = false;
+                evaluate();
+                sync_read();
+                d = read_diag(hs);
+            }
+        }
+        if (!d.finite) {
+            status = Status::nonfinite;
+            break;
+        }
+        if (opt.record_history) res.error_history.push_back(d.pk);
+        if (d.load <= load_tol && d.fw <= tol && d.pen <= tol) {
+            status = Status::converged;
+            u_fresh = true;
+            break;
+        }
+        const double merit = std::max(d.fw, d.pen);
+        if (opt.keep_best && merit < best_merit) {
+            best_merit = merit;
+            V::copy(N, P, bbest.p, s);
+            have_best = true;
+        }
+        if (merit < seen_merit * (1.0 - 1e-4)) {
+            seen_merit = merit;
+            stall = 0;
+        } else if (++stall >= stall_limit) {
+            if (forced_id < 3 && identification_step()) {
+                ++forced_id;
+                stall = 0;
+                continue;
+            }
+            status = Status::stagnated;
+            reason = "stall";
+            break;
+        }
+        if (d.nc >= 2) {
+            launch_cg();
+            pending = true;
+        } else if (!identification_step()) {
+            status = Status::nonpositive_curvature;
+            break;
+        }
+    }
+    if (pending) {
+        // the loop ended on the iteration budget right after a CG step
+        sync_read();
+        if (hs[S_OK] == 0.0 && !identification_step()) {
+            status = Status::nonpositive_curvature;
+            it = std::max(0, it - 1);
+        }
+    }
+
+    // final, freshly evaluated diagnostics on the returned pressure
+    if (status != Status::converged && opt.keep_best && have_best) {
+        const double merit = std::max(d.fw, d.pen);
+        if (!(merit <= best_merit)) {
+            V::copy(N, bbest.p, P, s);
+            u_fresh = false;
+            res.returned_best = true;
+        }
+    }
+    bbest.reset();
+    if (!u_fresh) apply(P, U);
+    evaluate();
+    sync_read();
+    d = read_diag(hs);
+    if (!d.finite && status != Status::nonfinite) status = Status::nonfinite;
+    if (status == Status::converged && !(d.load <= load_tol && d.fw <= tol && d.pen <= tol)) {
+        status = Status::verification_failed;
+        reason = "final_check";
+    }
+    if (status == Status::converged && req_tol < tol && !opt.allow_tolerance_relaxation) {
+        if (!(d.fw <= req_tol && d.pen <= req_tol)) {
+            status = Status::stagnated;
+            reason = "precision_limit";
+        }
+    }
+    {
+        static const int d3[3] = {S_R0, S_R1, S_R2};
+        V::objective(N, P, U, g0s, part, nblk, s);
+        finalize<Real>(part, nblk, 3, kS3, d3, scal, STEP_NONE, s);
+    }
+    res.pressure.alloc(N);
+    V::to_double(N, P, res.pressure.p, s);
+    if (!opt.light) {
+        res.displacement.alloc(N);
+        res.gap.alloc(N);
+        V::to_double(N, U, res.displacement.p, s);
+        V::to_double(N, G, res.gap.p, s);
+    } else {
+        res.displacement.reset();
+        res.gap.reset();
+    }
+    sync_read();
+    res.Ns = Ns;
+    res.objective = hs[S_R0] + datum * P_total;
+    res.approach = d.alpha + datum;
+    res.iterations = it;
+    res.contact_fraction = double(d.nc) / N;
+    res.mean_pressure = d.psum / N;
+    res.status = status;
+    res.status_reason = reason;
+    res.converged = status == Status::converged;
+    res.error = res.pk_error = d.pk;
+    res.fw_gap = d.G;
+    res.fw_error = d.fw;
+    res.load_error = d.load;
+    res.pressure_violation = 0.0;
+    res.penetration_error = d.pen;
+    res.requested_tol = req_tol;
+    res.effective_tol = tol;
+    res.p_ref = p_ref;
+    res.g_ref = g_ref;
+    res.matvec_count = mv;
+    res.precond_count = pcount;
+    res.precond_dropped = precond_dropped;
+    const long long rb = sizeof(Real);
+    res.mem_cg_state = 7 * N * rb;
+    res.mem_best = opt.keep_best ? N * rb : 0;
+    res.mem_mask = N;
+    res.mem_output = (rb == 8 ? 0 : N * 8) + (opt.light ? 0 : 2 * N * 8);
+    res.mem_peak = res.mem_cg_state + res.mem_best + res.mem_mask + res.mem_output;
+    res.time_matvec = tm_mv.seconds();
+    res.time_precond = tm_pc.seconds();
+    res.time_total = since(t_start);
+}
+
+template void solve_impl<double>(Context&, Operator&, const double*, double,
+                                 const SolveOptions&, Precond*, double*, Result&);
+template void solve_impl<float>(Context&, Operator&, const float*, float,
+                                const SolveOptions&, Precond*, float*, Result&);
+
+// ── nested driver (src/nested_solve.cpp:27-71,518-990; fft/periodic) ───────
+
+bool stencil_for_level(const std::string& engine, bool precond, double prev_occupancy,
+                       double occupancy_max) {
+    if (!precond) return false;
+    if (engine == "stencil") return true;
+    if (engine == "fft") return false;
+    return prev_occupancy < 0.0 || prev_occupancy < occupancy_max;
+}
+
+namespace {
+std::mutex g_cache_mu;
+std::map<std::tuple<int, int, double, double>, std::shared_ptr<Operator>> g_cache;
+} // namespace
+
+std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L, double E,
+                                          hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto key = std::make_tuple((int)kind, Ns, L, E);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return it->second;
+    auto op = std::make_shared<Operator>(kind, Ns, L, E, s);
+    op->build();
+    g_cache[key] = op;
+    return op;
+}
+
+void clear_operator_cache() {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache.clear();
+}
+
+void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double p_bar,
+                  double tol, int max_iter, bool use_pr, const NestedParams& np, Result& res,
+                  bool reuse_operators) {
+    // input validation before any allocation or iteration (src/nested_solve.cpp:523-617)
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (Ns <= 0) throw std::invalid_argument("solve_contact_nested: Ns must be positive");
+    if (!(L > 0.0) || !std::isfinite(L))
+        throw std::invalid_argument("solve_contact_nested: L must be positive");
+    if (!(E_star > 0.0) || !std::isfinite(E_star))
+        throw std::invalid_argument("solve_contact_nested: E_star must be positive");
+    const std::int64_t N64 = (std::int64_t)Ns * Ns;
+    if (N64 > (std::int64_t)std::numeric_limits<int>::max())
+        throw std::invalid_argument("solve_contact_nested: Ns*Ns exceeds the int index range");
+    if (!(p_bar > 0.0) || !std::isfinite(p_bar))
+        throw std::invalid_argument("solve_contact_nested: p_nominal must be positive");
+    if (!(tol > 0.0) || !std::isfinite(tol))
+        throw std::invalid_argument("solve_contact_nested: tol must be positive");
+    if (max_iter < 0) throw std::invalid_argument("solve_contact_nested: max_iter must be >= 0");
+    if (np.coarsest <= 0)
+        throw std::invalid_argument("solve_contact_nested: coarsest must be positive");
+    if (np.coarsest > Ns) throw std::invalid_argument("solve_contact_nested: coarsest > Ns");
+    if (np.q < 2) throw std::invalid_argument("solve_contact_nested: q must be >= 2");
+    if (np.leaf_side <= 0 || !pow2(np.leaf_side))
+        throw std::invalid_argument(
+            "solve_contact_nested: leaf_side must be a positive power of two");
+    if (!(np.coarse_tol > 0.0))
+        throw std::invalid_argument("solve_contact_nested: coarse_tol must be positive");
+    if (np.active_set) {
+        if (np.active_halo < 0)
+            throw std::invalid_argument("solve_contact_nested: active_halo must be >= 0");
+        if (np.active_max_rounds < 1)
+            throw std::invalid_argument("solve_contact_nested: active_max_rounds must be >= 1");
+        if (!(np.active_delta >= 0.0))
+            throw std::invalid_argument("solve_contact_nested: active_delta must be >= 0");
+        if (!(np.active_occupancy_max > 0.0))
+            throw std::invalid_argument(
+                "solve_contact_nested: active_occupancy_max must be > 0");
+    }
+    if (!(np.precond_occupancy_max > 0.0))
+        throw std::invalid_argument("solve_contact_nested: precond_occupancy_max must be > 0");
+    std::string policy = np.precision;
+    if (policy.empty()) policy = np.single_precision ? "float" : "double";
+    if (policy != "double" && policy != "float" && policy != "float_then_double")
+        throw std::invalid_argument(
+            "precision must be 'double', 'float', 'float_then_double' or ''");
+    const bool any_float = policy != "double";
+    if (!(np.float_floor > 0.0))
+        throw std::invalid_argument("solve_contact_nested: float_floor must be positive");
+    if (policy == "float_then_double" && np.active_set)
+        throw std::invalid_argument(
+            "solve_contact_nested: float_then_double does not yet route its polish through "
+            "the active-set path; use float_only or double_only with active_set=true");
+    std::vector<int> levels;
+    for (long long n = np.coarsest; n <= Ns; n *= 2) levels.push_back((int)n);
+    if (levels.empty() || levels.back() != Ns)
+        throw std::invalid_argument("solve_contact_nested: Ns must equal coarsest * 2^k");
+    if (np.backend != "h2" && np.backend != "fft" && np.backend != "periodic")
+        throw std::invalid_argument(
+            "solve_contact_nested: backend must be 'h2', 'fft', or 'periodic'");
+    if (np.active_set) {
+        if (np.backend != "h2")
+            throw std::invalid_argument(
+                "solve_contact_nested: active_set requires backend='h2' (masked matvec)");
+        if (levels.size() < 2)
+            throw std::invalid_argument(
+                "solve_contact_nested: active_set needs a coarse level (Ns > coarsest)");
+    }
+    if (np.backend == "h2")
+        throw std::invalid_argument(
+            "solve_contact_nested: backend 'h2' is not part of this build; use backend='fft' "
+            "(free space) or backend='periodic'");
+    if (np.precond_engine != "auto" && np.precond_engine != "stencil" &&
+        np.precond_engine != "fft")
+        throw std::invalid_argument("precond_engine must be 'auto', 'stencil' or 'fft'");
+
+    Context ctx;
+    hipStream_t s = ctx.stream;
+    const auto t_nested = Clock::now();
+    const int nblk = vec_blocks(N64);
+    {
+        static const int ops[3] = {RMIN, RMAX, RSUM};
+        static const int dst[3] = {S_R0, S_R1, S_R2};
+        VecOps<double>::stats(N64, gap_dev, ctx.part.p, nblk, s);
+        finalize<double>(ctx.part.p, nblk, 3, ops, dst, ctx.scal.p, STEP_NONE, s);
+        ctx.read_scal();
+    }
+    if (ctx.host_scal[S_R2] != 0.0)
+        throw std::invalid_argument("solve_contact_nested: gap contains non-finite values");
+    const double g_min_f = ctx.host_scal[S_R0], g_max_f = ctx.host_scal[S_R1];
+    const double datum = g_min_f + 0.5 * (g_max_f - g_min_f);
+    const double g_ref = std::max(g_max_f - g_min_f, p_bar * L / E_star);
+
+    const int ncoarse = (int)levels.size() - 1;
+    std::vector<DevBuf<double>> gap(ncoarse);
+    if (ncoarse > 0) {
+        gap[ncoarse - 1].alloc((std::size_t)levels[ncoarse - 1] * levels[ncoarse - 1]);
+        restrict2(Ns, gap_dev, gap[ncoarse - 1].p, s);
+        for (int i = ncoarse - 2; i >= 0; --i) {
+            gap[i].alloc((std::size_t)levels[i] * levels[i]);
+            restrict2(levels[i + 1], gap[i + 1].p, gap[i].p, s);
+        }
+    }
+
+    DevBuf<double> p_init;
+    bool have_init = false;
+    double t_build = 0.0, t_coarse = 0.0;
+    double prev_occupancy = -1.0;
+    std::vector<Stage> stages;
+    auto record_stage = [&](const std::string& name, const char* prec, double req, double eff,
+                            const Result& r) {
+        Stage st;
+        st.name = name;
+        st.precision = prec;
+        st.q = np.q;
+        st.requested_tol = req;
+        st.effective_tol = eff;
+        st.iterations = r.iterations;
+        st.matvec_count = r.matvec_count;
+        st.seconds = r.time_total;
+        st.status = r.status;
+        st.fw_error = r.fw_error;
+        st.penetration_error = r.penetration_error;
+        st.precond_dropped = r.precond_dropped;
+        stages.push_back(st);
+    };
+
+    const Operator::Kind kind =
+        np.backend == "fft" ? Operator::Kind::free_space : Operator::Kind::periodic;
+    for (std::size_t li = 0; li < levels.size(); ++li) {
+        const int n = levels[li];
+        const auto t_lvl = Clock::now();
+        std::shared_ptr<Operator> op;
+        if (reuse_operators) {
+            op = cached_operator(kind, n, L, E_star, s);
+        } else {
+            op = std::make_shared<Operator>(kind, n, L, E_star, s);
+            op->build();
+        }
+        const int r_lvl = std::min({np.precond_radius, std::max(1, n / 4), np.leaf_side});
+        const std::string engine =
+            (np.backend == "periodic" && np.precond_engine == "auto") ? std::string("fft")
+                                                                      : np.precond_engine;
+        const bool use_stencil =
+            stencil_for_level(engine, np.precond, prev_occupancy, np.precond_occupancy_max);
+        Precond pc(use_stencil ? Precond::Kind::stencil
+                               : (np.precond ? Precond::Kind::fourier : Precond::Kind::none),
+                   n, r_lvl);
+        t_build += since(t_lvl);
+
+        const bool finest = li + 1 == levels.size();
+        const double* glvl = finest ? gap_dev : gap[li].p;
+        const std::int64_t nn = (std::int64_t)n * n;
+        const bool polish_after = finest && policy == "float_then_double";
+        const bool level_float = any_float;
+        double lvl_tol = finest ? tol : np.coarse_tol;
+        if (polish_after) lvl_tol = np.float_floor;
+        else if (level_float) lvl_tol = std::max(lvl_tol, np.float_floor);
+        const bool light = finest ? np.light_result : true;
+        const bool record_history = finest && np.record_error_history;
+
+        SolveOptions lopt;
+        lopt.tol = lvl_tol;
+        lopt.requested_tol = finest ? tol : np.coarse_tol;
+        lopt.max_iter = max_iter;
+        lopt.use_pr = use_pr;
+        lopt.light = light;
+        lopt.record_history = record_history;
+        lopt.keep_best = !light;
+        lopt.allow_tolerance_relaxation = !finest || polish_after || np.allow_tolerance_relaxation;
+        lopt.g_ref = g_ref;
+        lopt.datum = datum;
+        lopt.datum_mode = level_float ? SolveOptions::Datum::caller : SolveOptions::Datum::solver;
+        lopt.precond_cost_gate = np.precond_cost_gate;
+
+        Result lres;
+        if (level_float) {
+            op->build_single();
+            // the datum is removed inside the cast, before any rounding
+            DevBuf<float> g0f(nn);
+            VecOps<float>::from_double(nn, glvl, datum, g0f.p, s);
+            if (!finest) gap[li].reset();
+            DevBuf<float> p0f;
+            if (have_init) {
+                p0f.alloc(nn);
+                VecOps<float>::from_double(nn, p_init.p, 0.0, p0f.p, s);
+                p_init.reset();
+            }
+            solve_impl<float>(ctx, *op, g0f.p, static_cast<float>(p_bar), lopt, &pc,
+                              have_init ? p0f.p : nullptr, lres);
+        } else {
+            solve_impl<double>(ctx, *op, glvl, p_bar, lopt, &pc,
+                               have_init ? p_init.p : nullptr, lres);
+            if (!finest) gap[li].reset();
+        }
+        prev_occupancy = lres.contact_fraction;
+        record_stage(std::string(finest ? "finest:" : "coarse:") + std::to_string(n),
+                     level_float ? "float" : "double", lopt.requested_tol, lopt.tol, lres);
+
+        if (polish_after) {
+            // float identified the contact at the float floor; polish in
+            // double, warm-started, to the requested tolerance
+            if (!reuse_operators) op->release_single();
+            DevBuf<double> p_warm = std::move(lres.pressure);
+            SolveOptions popt;
+            popt.tol = tol;
+            popt.requested_tol = tol;
+            popt.max_iter = max_iter;
+            popt.use_pr = use_pr;
+            popt.light = np.light_result;
+            popt.record_history = record_history;
+            popt.keep_best = !np.light_result;
+            popt.allow_tolerance_relaxation = np.allow_tolerance_relaxation;
+            popt.g_ref = g_ref;
+            popt.datum = datum;
+            popt.datum_mode = SolveOptions::Datum::solver;
+            popt.precond_cost_gate = np.precond_cost_gate;
+            Result pres;
+            solve_impl<double>(ctx, *op, glvl, p_bar, popt, &pc, p_warm.p, pres);
+            record_stage("polish:" + std::to_string(n), "double", tol, tol, pres);
+            pres.iterations += lres.iterations;
+            pres.matvec_count += lres.matvec_count;
+            pres.precond_count += lres.precond_count;
+            pres.time_matvec += lres.time_matvec;
+            pres.time_precond += lres.time_precond;
+            pres.mem_peak = std::max(pres.mem_peak, lres.mem_peak);
+            if (record_history) {
+                std::vector<double> h = lres.error_history;
+                h.insert(h.end(), pres.error_history.begin(), pres.error_history.end());
+                pres.error_history = std::move(h);
+            }
+            lres = std::move(pres);
+        }
+
+        if (!finest) {
+            t_coarse += lres.time_total;
+            p_init.alloc((std::size_t)4 * nn);
+            prolong2(n, lres.pressure.p, p_init.p, s);
+            have_init = true;
+        }
+        if (finest) res = std::move(lres);
+    }
+    ASPHER_HIP(hipStreamSynchronize(s));
+    res.stage_stats = std::move(stages);
+    res.time_build = t_build;
+    res.time_coarse = t_coarse;
+    res.time_total = since(t_nested);
+}
+
+} // namespace aspher

@@ -1,0 +1,119 @@
+// Host-side interface of the spectral (FFT-convolution) kernels.
+//
+// A 2-D spectral convolution y = F^-1 (s . F x) on an n x n grid is three
+// launches, each one HBM round trip:
+//   rows_fwd : real rows -> half spectrum A[iy][kx], kx in [0, n/2]
+//   cols     : per column kx: forward c2c along iy, multiply by the symbol,
+//              inverse c2c, in place in A (pruned: only rows_in rows read,
+//              only rows_out rows written)
+//   rows_inv : A rows -> real rows (only ncols_out values written)
+// The free-space operator runs it on the (2Ns)^2 zero-padded grid with
+// rows_in = rows_out = Ns (src/fft_operator.cpp:87-138); the periodic
+// operator and the Fourier preconditioner on the Ns^2 torus.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace aspher {
+
+enum class Symbol : int {
+    table = 0,     // real even kernel spectrum K[kx][min(ky, n-ky)] (free space)
+    periodic = 1,  // scale / |m| evaluated in double (periodic compliance)
+    precond = 2,   // |k| / Ns^2 evaluated in float (Fourier preconditioner)
+    build = 3,     // forward only: write Re/n^2 into the table, max |Im|, |Re|
+};
+
+enum class RowIn : int {
+    plain = 0,     // x[row*pitch + col]
+    masked = 1,    // mask ? x - gmean : 0 (gmean read from a device scalar)
+    embed = 2,     // wrap-around embedding of the Love table (K build)
+};
+
+enum class RowOut : int {
+    plain = 0,     // y = value
+    masked = 1,    // y = mask ? value : 0, per-block sum over the mask
+};
+
+// Launch configuration of one transform size (tuned by the autotuner).
+struct SpectralTuning {
+    int row_batch = 0;   // rows per workgroup in the row kernels (0 = auto)
+    int col_width = 0;   // columns per workgroup in the column kernel (0 = auto)
+    bool xcd_remap = true;
+};
+
+// Twiddle table exp(-2 pi i t/n), t in [0, n), built on the host in long
+// double and uploaded once per (n, precision).
+template <typename T>
+struct Twiddles {
+    int n = 0;
+    cx<T>* dev = nullptr;
+    explicit Twiddles(int n_);
+    ~Twiddles();
+    Twiddles(const Twiddles&) = delete;
+    Twiddles& operator=(const Twiddles&) = delete;
+};
+
+// A planned n x n spectral convolution in precision T. Owns the twiddles and
+// the half-spectrum workspace A (rows_alloc x pitch complex).
+template <typename T>
+class SpectralPlan {
+public:
+    SpectralPlan(int n, int rows_in, int cols_in, int rows_out, int cols_out,
+                 SpectralTuning tune = {});
+    ~SpectralPlan();
+    SpectralPlan(const SpectralPlan&) = delete;
+    SpectralPlan& operator=(const SpectralPlan&) = delete;
+
+    // Full convolution: y = inv(sym . fwd(x)). x and y may alias.
+    // ktab: symbol table (Symbol::table) with row pitch ktab_pitch.
+    // mask/scal/part: for RowIn::masked / RowOut::masked (gmean at
+    // scal[gmean_idx]; per-block partial sums written to part[block]).
+    void convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbol sym,
+                  const T* ktab, int ktab_pitch, double sym_scale,
+                  hipStream_t s, RowIn in_mode = RowIn::plain,
+                  RowOut out_mode = RowOut::plain,
+                  const std::uint8_t* mask = nullptr,
+                  const double* scal = nullptr, int gmean_idx = 0,
+                  double* part = nullptr);
+
+    // Forward-only pass used to build the free-space kernel spectrum from
+    // the Love table (T = double): writes Re/n^2 into kout[(kx)(n/2+1)+ky]
+    // for ky in [0, n/2] and per-block max |Im|, max |Re| into part.
+    void build_spectrum(const double* table, int Ns, T* kout, double* part,
+                        hipStream_t s);
+
+    int row_out_blocks() const;   // blocks of rows_inv (partial count)
+    int col_blocks() const;       // blocks of cols (partial count, build)
+    std::size_t workspace_bytes() const;
+
+    int n, rows_in, cols_in, rows_out, cols_out, nh, pitch;
+
+private:
+    void rows_fwd(const T* x, int x_pitch, RowIn mode, const std::uint8_t* mask,
+                  const double* scal, int gmean_idx, const double* table,
+                  int Ns_embed, int nrows, int ncols, hipStream_t s);
+    void cols(Symbol sym, const T* ktab, int ktab_pitch, double sym_scale,
+              T* kout, double* part, int rin, int rout, hipStream_t s);
+    void rows_inv(T* y, int y_pitch, RowOut mode, const std::uint8_t* mask,
+                  double* part, hipStream_t s);
+
+    std::unique_ptr<Twiddles<T>> tw_;
+    cx<T>* A_ = nullptr;
+    std::size_t a_rows_ = 0;
+    bool pow2_ = false;
+    int logL_row_ = 0, logn_ = 0;
+    int row_batch_ = 1, col_width_ = 1;
+    bool xcd_remap_ = true;
+    std::vector<int> rad_row_, rad_col_;
+};
+
+// Power-of-two test and the mixed-radix factorisation used by the generic
+// (non-power-of-two) kernels.
+bool is_pow2(int n);
+std::vector<int> factorize(int L);
+
+} // namespace aspher

@@ -1,0 +1,57 @@
+// GPU-time accounting for the solver phases.
+#pragma once
+
+#include "common.hpp"
+
+#include <utility>
+#include <vector>
+
+namespace aspher {
+
+// Accumulates GPU time of (start, stop) event pairs without synchronising:
+// completed pairs are drained whenever the host has synchronised anyway.
+class EventTimer {
+public:
+    ~EventTimer() {
+        for (auto& pr : pending_) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+        for (auto e : pool_) (void)hipEventDestroy(e);
+    }
+    void start(hipStream_t s) {
+        cur_ = get();
+        (void)hipEventRecord(cur_, s);
+    }
+    void stop(hipStream_t s) {
+        hipEvent_t e = get();
+        (void)hipEventRecord(e, s);
+        pending_.emplace_back(cur_, e);
+    }
+    // call after a stream synchronisation
+    void drain() {
+        for (auto& pr : pending_) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) total_ += ms * 1e-3;
+            pool_.push_back(pr.first);
+            pool_.push_back(pr.second);
+        }
+        pending_.clear();
+    }
+    double seconds() const { return total_; }
+
+private:
+    hipEvent_t get() {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        ASPHER_HIP(hipEventCreate(&e));
+        return e;
+    }
+    std::vector<hipEvent_t> pool_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_;
+    hipEvent_t cur_ = nullptr;
+    double total_ = 0.0;
+};
+
+} // namespace aspher

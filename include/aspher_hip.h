@@ -1,0 +1,202 @@
+/*
+ * C ABI of the MI355X spectral contact library (libaspher_hip.so).
+ *
+ * Plain pointers, sizes and status codes; no C++ or framework types cross
+ * this boundary. Each entry point replaces one call of the reference's
+ * pybind11 layer (cofebem/aspher python/bindings.cpp), cited per function,
+ * and keeps its argument meaning, defaults and error behaviour:
+ *   invalid input (std::invalid_argument -> ValueError)  ASPHER_ERR_INVALID
+ *   lifecycle misuse (std::logic_error -> RuntimeError)    ASPHER_ERR_LOGIC
+ *   broken construction (std::runtime_error)               ASPHER_ERR_RUNTIME
+ *   HIP/device failure                                     ASPHER_ERR_DEVICE
+ * Numerical outcomes are never errors: they are the `status` string of the
+ * result (converged, stagnated, max_iterations, nonpositive_curvature,
+ * nonfinite, verification_failed, resource_limit).
+ *
+ * Grids are row-major, flat index iy*Ns + ix, FP64 on the host side. Calls
+ * are synchronous; a handle must not be used from two threads at once.
+ */
+#ifndef ASPHER_HIP_H
+#define ASPHER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASPHER_OK 0
+#define ASPHER_ERR_INVALID 1
+#define ASPHER_ERR_LOGIC 2
+#define ASPHER_ERR_RUNTIME 3
+#define ASPHER_ERR_DEVICE 4
+
+typedef struct aspher_solver aspher_solver;
+typedef struct aspher_result aspher_result;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* aspher_last_error(void);
+
+/* Library version string. */
+const char* aspher_version(void);
+
+/* Number of visible HIP devices (0 without a GPU; never an error). */
+int aspher_device_count(int* count);
+
+/* ContactSolver.__init__ (python/bindings.cpp:117-162, defaults :804-815).
+ * backend: "fft" (free-space, FP64 Love table + padded FFT), "periodic",
+ * "dense" (direct Love sum on the device). Other reference backends
+ * ("h2", "hmatrix", "") are not part of this library: ASPHER_ERR_INVALID. */
+int aspher_solver_create(int grid_size, double domain_size, double E_star,
+                         const char* backend, aspher_solver** out);
+void aspher_solver_destroy(aspher_solver* s);
+
+/* ContactSolver.matvec (python/bindings.cpp:164-184): u = S p, N = Ns^2
+ * host doubles in and out. */
+int aspher_solver_matvec(aspher_solver* s, const double* p, double* u);
+
+/* Same on device buffers already resident in HBM: x, y hold Ns^2 values of
+ * FP64 (precision 64) or FP32 (precision 32); x and y may alias. */
+int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int precision);
+
+/* Average device time (ms) of `reps` back-to-back device matvecs, measured
+ * with HIP events on the library's own stream (benchmark support). */
+int aspher_solver_time_matvec(aspher_solver* s, const void* x, void* y, int precision,
+                              int reps, double* ms_per_matvec);
+
+/* ContactSolver.solve (python/bindings.cpp:186-234, defaults :818-828):
+ * precond "none" | "" | "fourier" (13-tap stencil) | "fourier-fft" (full-grid
+ * |k|); p_init may be NULL. */
+int aspher_solver_solve(aspher_solver* s, const double* gap, double p_nominal, double tol,
+                        int max_iter, int use_pr, const char* precond,
+                        const double* p_init, aspher_result** out);
+
+/* Same with the gap (and optional p_init) already on the device (FP64). */
+int aspher_solver_solve_device(aspher_solver* s, const double* gap_dev, double p_nominal,
+                               double tol, int max_iter, int use_pr, const char* precond,
+                               const double* p_init_dev, aspher_result** out);
+
+/* ContactSolver.hmatrix_info for fft / periodic / dense
+ * (python/bindings.cpp:307-328). */
+typedef struct {
+    char backend[16];
+    int64_t n;
+    int padded_side;         /* 2 Ns for fft, Ns otherwise */
+    int64_t bytes;           /* spectrum + scratch held */
+    int64_t bytes_spectrum;
+    int64_t bytes_scratch;
+    double build_seconds;
+} aspher_operator_info;
+int aspher_solver_info(aspher_solver* s, aspher_operator_info* out);
+
+/* solve_nested keyword arguments (python/bindings.cpp:836-853). */
+typedef struct {
+    int coarsest;                   /* 64 */
+    int q;                          /* 6 (accepted, unused: no H2 here) */
+    int leaf_side;                  /* 8 */
+    int precond;                    /* 1 */
+    char precond_engine[16];        /* "auto" | "stencil" | "fft" */
+    int precond_radius;             /* 2 */
+    double precond_occupancy_max;   /* 0.4 */
+    double precond_cost_gate;       /* 0.0 */
+    double coarse_tol;              /* 1e-4 */
+    int single_precision;           /* 0 */
+    char precision[24];             /* "" | "double" | "float" | "float_then_double" */
+    double float_floor;             /* 2e-6 */
+    int allow_tolerance_relaxation; /* 0 */
+    int light_result;               /* 0 */
+    char backend[16];               /* "h2" (reference default; not in this build) */
+    int record_error_history;       /* 0 */
+    int active_set;                 /* 0 */
+    double active_delta;            /* 0.05 */
+    int active_halo;                /* 2 */
+    int active_max_rounds;          /* 5 */
+    int active_all_levels;          /* 1 */
+    double active_occupancy_max;    /* 0.4 */
+    int reuse_operators;            /* 0: rebuild every level like the reference;
+                                       1: keep built operators in a process cache */
+} aspher_nested_params;
+void aspher_nested_params_default(aspher_nested_params* p);
+
+/* solve_nested (python/bindings.cpp:362-432): gap is Ns^2 host doubles. */
+int aspher_solve_nested(int grid_size, const double* gap, double p_nominal,
+                        double domain_size, double E_star, double tol, int max_iter,
+                        int use_pr, const aspher_nested_params* params,
+                        aspher_result** out);
+int aspher_solve_nested_device(int grid_size, const double* gap_dev, double p_nominal,
+                               double domain_size, double E_star, double tol, int max_iter,
+                               int use_pr, const aspher_nested_params* params,
+                               aspher_result** out);
+
+/* ContactResult scalar properties (python/bindings.cpp:637-802). */
+typedef struct {
+    int grid_size;
+    double objective, contact_area, mean_pressure, approach;
+    int iterations;
+    double error;
+    int converged;
+    int active_rounds, active_fallback;
+    char status[32];
+    char status_reason[32];
+    double pk_error, fw_gap, fw_error, load_error, pressure_violation, penetration_error;
+    double requested_tol, effective_tol, p_ref, g_ref;
+    char validation_scope[32];
+    char operator_error_kind[32];
+    int64_t matvec_count, verification_matvec_count, precond_count;
+    int identification_steps, returned_best, precond_dropped;
+    int64_t mem_cg_state, mem_best_iterate, mem_contact_mask, mem_output_arrays, mem_peak;
+    double t_total, t_matvec, t_precond, t_build, t_coarse, t_verification, t_candidate,
+        t_output;
+    int has_displacement, has_gap;
+    int n_history, n_stages;
+} aspher_result_info;
+
+int aspher_result_get_info(const aspher_result* r, aspher_result_info* out);
+/* which: 0 pressure, 1 displacement, 2 gap; out receives Ns^2 doubles.
+ * ASPHER_ERR_INVALID when the field is absent (light result). */
+int aspher_result_get_field(const aspher_result* r, int which, double* out);
+/* device pointer of a field (valid until aspher_result_destroy) */
+int aspher_result_field_device(const aspher_result* r, int which, const double** dev);
+int aspher_result_get_history(const aspher_result* r, double* out);
+
+typedef struct {
+    char name[48];
+    char precision[16];
+    int q;
+    double requested_tol, effective_tol;
+    int iterations;
+    int64_t matvec_count;
+    double seconds;
+    char status[32];
+    double fw_error, penetration_error;
+    int precond_dropped;
+} aspher_stage_info;
+int aspher_result_get_stage(const aspher_result* r, int index, aspher_stage_info* out);
+void aspher_result_destroy(aspher_result* r);
+
+/* configure_allocator (python/bindings.cpp, runtime_policy): host allocator
+ * policy; kept for API parity, returns 0 (not applied) through *applied. */
+int aspher_configure_allocator(int64_t mmap_threshold_bytes, int64_t trim_threshold_bytes,
+                               int* applied);
+
+/* Kernel-level entry points used by the parity tests. */
+/* love_uz at n points on the device (src/boussinesq_kernel.cpp:71-95) */
+int aspher_love_points(int n, const double* x, const double* y, const double* a,
+                       const double* b, double* out);
+/* Ns x Ns FP64 influence table (src/boussinesq_kernel.cpp:97-106) */
+int aspher_love_table(int grid_size, double domain_size, double E_star, double* out);
+/* stencil taps (src/stencil_precond.cpp:199-220); arrays hold >= (2R+1)^2 */
+int aspher_stencil_taps(int grid_size, int radius, int* ntaps, int* dx, int* dy, double* w);
+/* z = M^-1 g on the contact mask (src/stencil_precond.cpp:63-109,
+ * src/fourier_precond.cpp:50-116): kind "fourier" (stencil) | "fourier-fft";
+ * precision 64 or 32 (host arrays are FP64 either way). */
+int aspher_precond_apply(int grid_size, const char* kind, int radius, int precision,
+                         const double* g, const uint8_t* mask, double* z);
+/* Drop the process-wide operator cache. */
+int aspher_clear_cache(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASPHER_HIP_H */

@@ -1,0 +1,872 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement (numpy / scipy.fft, plus the C Love kernel in ``love.c``) of
+the reference's spectral contact path. Only ``tests/``, ``__graft_entry__.smoke``
+and the ``cpu_baseline`` leg of ``bench.py`` may import this module, and only as
+the checker or as the timed CPU baseline; the product path (``aspher_amd``)
+never imports, calls or links it.
+
+Every function cites the reference file:line it restates. Transforms use
+scipy.fft (pocketfft, the same FFT library the reference vendors in
+third_party/pocketfft); the reference's engine is unnormalised
+(fwd∘inv = n², src/fft_engine.hpp:18-22), so ``_inv`` multiplies scipy's
+normalised inverse back by n².
+
+Pinning (what checks each function, see tests/test_oracle_*.py):
+  love_uz / love_table     80-digit fixture tests/golden/love_reference.txt,
+                           self-term closed form, parity, homogeneity
+  FreeSpaceOperator        brute-force dense convolution (rel <1e-12),
+                           K̂ realness, symmetry
+  PeriodicOperator         cos-mode eigenvalues 2/(E*|q|), DC -> 0,
+                           symmetric/PSD/circulant on odd grids
+  StencilPreconditioner    universal-constant tap table of
+                           tests/test_stencil.cpp:21-33 (±5e-6), symbol > 0
+  FourierPreconditioner    offset invariance and symmetry (<1e-12)
+  project_load_simplex     sorting projector, idempotence
+  solve_contact            manufactured-KKT exact p*, support enumeration,
+                           Westergaard closed form, FW bound identities
+  solve_nested             nested == single level, datum invariance
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+import scipy.fft as sfft
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build_lib(force: bool = False) -> str:
+    """Compile love.c into liboracle.so (gcc, -O2, no fast-math)."""
+    src = os.path.join(_HERE, "love.c")
+    if force or not os.path.exists(_SO) or os.path.getmtime(_SO) < os.path.getmtime(src):
+        tmp = _SO + ".tmp%d" % os.getpid()
+        subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11",
+                               "-fno-fast-math", src, "-o", tmp, "-lm"])
+        os.replace(tmp, _SO)
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = _SO if os.path.exists(_SO) else build_lib()
+        L = ctypes.CDLL(path)
+        d = ctypes.c_double
+        pd = ctypes.POINTER(ctypes.c_double)
+        L.oracle_love_uz.restype = d
+        L.oracle_love_uz.argtypes = [d, d, d, d]
+        L.oracle_asinh_diff.restype = d
+        L.oracle_asinh_diff.argtypes = [d, d, d]
+        L.oracle_love_many.restype = None
+        L.oracle_love_many.argtypes = [ctypes.c_int64, pd, pd, pd, pd, pd]
+        L.oracle_love_table.restype = None
+        L.oracle_love_table.argtypes = [ctypes.c_int, d, d, pd]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ── L1: Love kernel (src/boussinesq_kernel.cpp:44-106) ──────────────────────
+
+def love_uz(x, y, a, b):
+    """Vectorised love_uz (src/boussinesq_kernel.cpp:71-95)."""
+    x, y, a, b = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64) for v in (x, y, a, b)))
+    x = np.ascontiguousarray(x).ravel(); y = np.ascontiguousarray(y).ravel()
+    a = np.ascontiguousarray(a).ravel(); b = np.ascontiguousarray(b).ravel()
+    out = np.empty_like(x)
+    lib().oracle_love_many(x.size, _ptr(x), _ptr(y), _ptr(a), _ptr(b), _ptr(out))
+    return out
+
+
+def love_table(Ns: int, L: float = 1.0, E_star: float = 1.0) -> np.ndarray:
+    """(Ns, Ns) table T[dy, dx] (src/boussinesq_kernel.cpp:97-106)."""
+    T = np.empty((Ns, Ns), dtype=np.float64)
+    lib().oracle_love_table(int(Ns), float(L), float(E_star), _ptr(T))
+    return T
+
+
+def dense_matrix(Ns: int, L: float = 1.0, E_star: float = 1.0) -> np.ndarray:
+    """N x N dense influence matrix (src/boussinesq_kernel.cpp:108-115,
+    entry() at include/boussinesq_kernel.hpp:37-41)."""
+    T = love_table(Ns, L, E_star)
+    ix = np.arange(Ns * Ns) % Ns
+    iy = np.arange(Ns * Ns) // Ns
+    return T[np.abs(iy[:, None] - iy[None, :]), np.abs(ix[:, None] - ix[None, :])]
+
+
+# ── transforms (src/fft_engine.hpp: unnormalised r2c / c2r) ─────────────────
+
+def _fwd(G):
+    return sfft.rfft2(G)
+
+
+def _inv(C, n):
+    return sfft.irfft2(C, s=(n, n)) * (n * n)
+
+
+# ── L3: free-space operator (src/fft_operator.cpp:18-163) ───────────────────
+
+class FreeSpaceOperator:
+    """Hockney zero-padded (2Ns)^2 circular convolution with the Love table."""
+
+    def __init__(self, Ns: int, L: float = 1.0, E_star: float = 1.0, table=None):
+        self.Ns, self.M = Ns, 2 * Ns
+        self.table = love_table(Ns, L, E_star) if table is None else table
+        self.build()
+
+    def embed(self) -> np.ndarray:
+        """Wrap-around embedding (src/fft_operator.cpp:30-44): offset d at
+        index d (d>=0) or M+d (d<0); row/column Ns zero."""
+        Ns, M = self.Ns, self.M
+        idx = np.arange(M)
+        ad = np.where(idx < Ns, idx, M - idx)
+        K = self.table[np.minimum(ad, Ns - 1)[:, None], np.minimum(ad, Ns - 1)[None, :]].copy()
+        K[Ns, :] = 0.0
+        K[:, Ns] = 0.0
+        return K
+
+    def build(self):
+        """FFT of the embedding, realness check, K̂ = Re/M^2
+        (src/fft_operator.cpp:46-68)."""
+        M = self.M
+        Kc = _fwd(self.embed())                    # (M, Ns+1) [ky, kx]
+        max_im = np.abs(Kc.imag).max()
+        max_re = np.abs(Kc.real).max()
+        if max_im > 1e-10 * max_re:
+            raise RuntimeError("FFTOperator: kernel spectrum has a non-negligible "
+                               "imaginary part; wrap-around embedding is broken")
+        self.Kh = Kc.real * (1.0 / (float(M) * float(M)))
+        self.Kh_f = self.Kh.astype(np.float32)     # src/fft_operator.cpp:71-78
+
+    def matvec(self, x, dtype=np.float64):
+        """scatter -> fwd -> xK̂ -> inv -> gather (src/fft_operator.cpp:87-138)."""
+        Ns, M = self.Ns, self.M
+        G = np.zeros((M, M), dtype=dtype)
+        G[:Ns, :Ns] = np.asarray(x, dtype=dtype).reshape(Ns, Ns)
+        C = _fwd(G)
+        C *= (self.Kh if dtype == np.float64 else self.Kh_f)
+        return _inv(C, M)[:Ns, :Ns].astype(dtype).ravel()
+
+
+# ── L3: periodic operator (src/periodic_fft_operator.cpp:11-177) ────────────
+
+class PeriodicOperator:
+    """u_hat = scale/|m| p_hat, scale = L/(pi E* Ns^2), zero mode -> 0."""
+
+    def __init__(self, Ns: int, L: float = 1.0, E_star: float = 1.0):
+        if Ns < 2:
+            raise ValueError("grid_size must be at least 2")
+        self.Ns = Ns
+        self.scale = L / (math.pi * E_star * float(Ns) * Ns)
+        nh = Ns // 2 + 1
+        ky = np.arange(Ns)
+        my = np.where(ky < (Ns + 1) // 2, ky, ky - Ns).astype(np.float64)
+        kx = np.arange(nh).astype(np.float64)
+        m2 = kx[None, :] ** 2 + (my ** 2)[:, None]
+        with np.errstate(divide="ignore"):
+            W = np.where(m2 > 0, self.scale / np.sqrt(m2), 0.0)
+        self.W = W                                   # symbol in double, cast per precision
+        self.W_f = W.astype(np.float32)
+
+    def matvec(self, x, dtype=np.float64):
+        Ns = self.Ns
+        C = _fwd(np.asarray(x, dtype=dtype).reshape(Ns, Ns))
+        C *= (self.W if dtype == np.float64 else self.W_f)
+        return _inv(C, Ns).astype(dtype).ravel()
+
+
+class DenseOperator:
+    """Dense matvec from the Love table (the reference's 'dense' backend)."""
+
+    def __init__(self, Ns: int, L: float = 1.0, E_star: float = 1.0):
+        self.Ns = Ns
+        self.S = dense_matrix(Ns, L, E_star)
+
+    def matvec(self, x, dtype=np.float64):
+        if dtype == np.float64:
+            return self.S @ np.asarray(x, dtype=np.float64)
+        return (self.S @ np.asarray(x, dtype=np.float64)).astype(dtype)
+
+
+# ── L4: preconditioners ─────────────────────────────────────────────────────
+
+def stencil_kernel_grid(Nw: int) -> np.ndarray:
+    """w = F^-1 |k| (DC 0) normalised by w(0,0) (src/stencil_precond.cpp:19-43)."""
+    nh = Nw // 2 + 1
+    ky = np.arange(Nw)
+    kof = np.where(ky < Nw // 2, ky, ky - Nw).astype(np.float64)
+    kx = np.arange(nh).astype(np.float64)
+    C = np.hypot(kx[None, :], kof[:, None]).astype(np.complex128)
+    C[0, 0] = 0.0
+    G = _inv(C, Nw)
+    return G / G[0, 0]                               # [iy, ix]
+
+
+class StencilPreconditioner:
+    """13-tap (R=2) real-space |k| stencil (src/stencil_precond.cpp:53-109,199-220)."""
+
+    def __init__(self, Ns: int, radius: int = 2):
+        if radius < 1:
+            raise ValueError("stencil radius must be >= 1")
+        if 2 * radius + 1 > Ns:
+            raise ValueError("stencil radius too large for the grid")
+        self.Ns, self.R = Ns, radius
+        Nw = Ns if Ns < 512 else 512
+        w = stencil_kernel_grid(Nw)
+        self.dx, self.dy, self.w = [], [], []
+        for dy in range(-radius, radius + 1):
+            for dx in range(-radius, radius + 1):
+                if dx * dx + dy * dy > radius * radius:
+                    continue
+                self.dx.append(dx)
+                self.dy.append(dy)
+                self.w.append(float(w[dy % Nw, dx % Nw]))
+
+    def apply(self, g, contact, dtype=None):
+        """apply_full (src/stencil_precond.cpp:63-109)."""
+        Ns = self.Ns
+        dtype = np.dtype(g.dtype if dtype is None else dtype).type
+        c = np.asarray(contact).astype(bool).reshape(Ns, Ns)
+        gd = g.astype(np.float64).reshape(Ns, Ns)
+        ng = int(c.sum())
+        gmean = gd[c].sum() / ng if ng else 0.0
+        src = np.where(c, gd - gmean, 0.0)
+        acc = np.zeros((Ns, Ns))
+        for dx, dy, w in zip(self.dx, self.dy, self.w):
+            # value at (ix+dx, iy+dy) with periodic wrap
+            acc += w * np.roll(np.roll(src, -dy, axis=0), -dx, axis=1)
+        z = np.where(c, acc, 0.0).astype(dtype)
+        nc = ng
+        if nc:
+            zsum = acc[c].sum()
+            zmean = dtype(zsum / nc)
+            z[c] -= zmean
+        return z.ravel()
+
+
+class FourierPreconditioner:
+    """Full-grid |k| preconditioner, symbol in float (src/fourier_precond.cpp:12-116)."""
+
+    def __init__(self, Ns: int):
+        self.Ns = Ns
+        nh = Ns // 2 + 1
+        norm = np.float32(1.0) / (np.float32(Ns) * np.float32(Ns))
+        ky = np.arange(Ns)
+        kof = np.where(ky < Ns // 2, ky, ky - Ns).astype(np.float32)
+        kx = np.arange(nh).astype(np.float32)
+        wh = norm * np.hypot(kx[None, :], kof[:, None]).astype(np.float32)
+        wh[0, 0] = 0.0
+        self.wh = wh.astype(np.float32)              # [ky, kx]
+
+    def apply(self, g, contact, dtype=None):
+        Ns = self.Ns
+        dtype = np.dtype(g.dtype if dtype is None else dtype).type
+        c = np.asarray(contact).astype(bool).ravel()
+        ng = int(c.sum())
+        gsum = g[c].astype(np.float64).sum()
+        gmean = dtype(gsum / ng) if ng else dtype(0)
+        G = np.where(c, g - gmean, dtype(0)).astype(dtype).reshape(Ns, Ns)
+        C = _fwd(G)
+        C *= self.wh.astype(dtype)
+        Gi = _inv(C, Ns).astype(dtype).ravel()
+        z = np.where(c, Gi, dtype(0)).astype(dtype)
+        nc = ng
+        if nc:
+            zsum = Gi[c].astype(np.float64).sum()
+            zmean = dtype(zsum / nc)
+            z[c] -= zmean
+        return z
+
+
+def stencil_for_level(engine: str, precond: bool, prev_occupancy: float,
+                      occupancy_max: float) -> bool:
+    """src/nested_solve.cpp:27-39."""
+    if not precond:
+        return False
+    if engine == "stencil":
+        return True
+    if engine == "fft":
+        return False
+    return prev_occupancy < 0.0 or prev_occupancy < occupancy_max
+
+
+# ── L5: load-simplex projection (src/contact_solver.cpp:31-60) ──────────────
+
+def project_load_simplex(y: np.ndarray, P_total: float) -> float:
+    """Michelot threshold iteration, in place; returns theta."""
+    N = y.size
+    if N == 0:
+        return 0.0
+    yd = y.astype(np.float64)
+    ssum = yd.sum()
+    nact = N
+    theta = (ssum - P_total) / nact
+    for _ in range(64):
+        m = yd > theta
+        n2 = int(m.sum())
+        if n2 == 0 or n2 == nact:
+            break
+        nact = n2
+        ssum = yd[m].sum()
+        theta = (ssum - P_total) / nact
+    th = y.dtype.type(theta)
+    y[:] = np.where(y > th, y - th, y.dtype.type(0))
+    return theta
+
+
+# ── L5: certified Polonsky–Keer CG (src/contact_solver.cpp:150-587) ─────────
+
+STATUSES = ("converged", "stagnated", "max_iterations", "nonpositive_curvature",
+            "nonfinite", "verification_failed", "resource_limit")
+
+
+@dataclass
+class SolveOptions:
+    """include/contact_solver.hpp:151-191 (+ SolveScales :134-147)."""
+    tol: float = 1e-8
+    requested_tol: float = 0.0
+    max_iter: int = 5000
+    use_pr: bool = True
+    light: bool = False
+    record_history: bool = False
+    keep_best: bool = True
+    load_tol: float = 0.0
+    stall_limit: int = 200
+    allow_tolerance_relaxation: bool = False
+    p_ref: float = 0.0
+    g_ref: float = 0.0
+    datum_mode: str = "automatic"          # automatic | solver | caller
+    datum: float = 0.0
+
+
+@dataclass
+class Result:
+    pressure: np.ndarray = None
+    displacement: Optional[np.ndarray] = None
+    gap: Optional[np.ndarray] = None
+    approach: float = 0.0
+    objective: float = 0.0
+    error: float = 0.0
+    iterations: int = 0
+    converged: bool = False
+    contact_fraction: float = 0.0
+    mean_pressure: float = 0.0
+    error_history: List[float] = field(default_factory=list)
+    status: str = "max_iterations"
+    status_reason: str = ""
+    pk_error: float = 0.0
+    fw_gap: float = 0.0
+    fw_error: float = 0.0
+    load_error: float = 0.0
+    pressure_violation: float = 0.0
+    penetration_error: float = 0.0
+    requested_tol: float = 0.0
+    effective_tol: float = 0.0
+    p_ref: float = 0.0
+    g_ref: float = 0.0
+    matvec_count: int = 0
+    precond_count: int = 0
+    identification_steps: int = 0
+    returned_best: bool = False
+    time_total: float = 0.0
+    stage_stats: list = field(default_factory=list)
+
+
+class _Diag:
+    __slots__ = ("alpha", "gmin", "pk", "G", "fw", "pen", "load", "psum", "pv", "nc", "finite")
+
+
+def solve_contact(S: Callable, g0: np.ndarray, p_bar, opt: SolveOptions = None,
+                  precond: Optional[Callable] = None, p_init=None,
+                  dtype=np.float64) -> Result:
+    """Restatement of solve_contact_impl<Real> (src/contact_solver.cpp:150-587).
+
+    S(x) -> S x in `dtype`; precond(g, contact_bool) -> z in `dtype`.
+    All reductions accumulate in float64 like the reference."""
+    opt = opt or SolveOptions()
+    R = np.dtype(dtype).type
+    is_double = (np.dtype(dtype) == np.float64)
+    g0 = np.ascontiguousarray(g0, dtype=dtype).ravel()
+    N = g0.size
+    p_bar = R(p_bar)
+    if N == 0 or not (p_bar > 0):
+        raise ValueError("solve_contact: empty gap or p_bar <= 0")
+    if not (opt.tol > 0.0):
+        raise ValueError("solve_contact: tol must be positive")
+    if opt.max_iter < 0:
+        raise ValueError("solve_contact: max_iter must be >= 0")
+    if not np.all(np.isfinite(g0)):
+        raise ValueError("solve_contact: g0 contains non-finite values")
+
+    P_total = float(p_bar) * N
+    gmin_v = float(g0.min())
+    gmax_v = float(g0.max())
+    g_ref = opt.g_ref
+    if not (g_ref > 0.0):
+        g_ref = gmax_v - gmin_v
+    if not (g_ref > 0.0):
+        g_ref = 1.0
+    p_ref = opt.p_ref if opt.p_ref > 0.0 else float(p_bar)
+    load_tol = opt.load_tol if opt.load_tol > 0.0 else (1e-12 if is_double else 5e-7)
+    tol = opt.tol
+    req_tol = opt.requested_tol if opt.requested_tol > 0.0 else tol
+    if opt.datum_mode == "automatic":
+        datum = gmin_v + 0.5 * (gmax_v - gmin_v); subtract = True
+    elif opt.datum_mode == "solver":
+        datum = opt.datum; subtract = True
+    else:
+        datum = opt.datum; subtract = False
+    gshift = R(datum) if subtract else R(0)
+    g0s = (g0 - gshift).astype(dtype)               # (g0(i) - gshift) in Real
+
+    if p_init is not None:
+        p = np.array(p_init, dtype=dtype).ravel()
+        if p.size != N:
+            raise ValueError("solve_contact: p_init size mismatch")
+        p = np.maximum(p, R(0))
+        s = p.astype(np.float64).sum()
+        if s > 0.0:
+            p = (p * R(P_total / s)).astype(dtype)
+        else:
+            p = np.full(N, p_bar, dtype=dtype)
+    else:
+        p = np.full(N, p_bar, dtype=dtype)
+    t = np.zeros(N, dtype=dtype)
+    u = np.zeros(N, dtype=dtype)
+    g = np.zeros(N, dtype=dtype)
+    g_prev = np.zeros(N, dtype=dtype)
+    z = np.zeros(N, dtype=dtype)
+    r = np.zeros(N, dtype=dtype)
+    best_p = None
+    res = Result()
+    cnt = {"mv": 0, "pc": 0}
+    t_start = time.perf_counter()
+
+    def apply(x):
+        cnt["mv"] += 1
+        return np.asarray(S(x), dtype=dtype).ravel()
+
+    st = {"G_old": 1.0, "delta": R(0), "s_id": 0.0, "u_fresh": False}
+
+    def evaluate():
+        nonlocal g
+        d = _Diag()
+        v = (u + g0s).astype(dtype)
+        g = v.copy()
+        vd = v.astype(np.float64)
+        pos = p > 0
+        nc = int(pos.sum())
+        vmin = float(vd.min())
+        gsum = float(vd[pos].sum())
+        d.nc = nc
+        d.alpha = gsum / nc if nc else vmin
+        d.gmin = vmin - d.alpha
+        alpha = R(d.alpha)
+        gmin = R(d.gmin)
+        pd = p.astype(np.float64)
+        pv = float((pd * g.astype(np.float64)).sum())
+        g = (g - alpha).astype(dtype)
+        gd = g.astype(np.float64)
+        gcs = float(gd[pos].sum())
+        e = float((pd * np.abs(gd)).sum())
+        Gc = float((pd * (g - gmin).astype(dtype).astype(np.float64)).sum())
+        psum = float(pd.sum())
+        if d.nc > 0:
+            delta = gcs / d.nc
+            d.alpha += delta
+            d.gmin -= delta
+        d.pk = e / (P_total * g_ref)
+        d.G = Gc
+        d.fw = Gc / (P_total * g_ref)
+        d.pen = max(0.0, -d.gmin) / g_ref
+        d.psum = psum
+        d.pv = pv
+        d.load = abs(psum - P_total) / P_total
+        d.finite = all(math.isfinite(q) for q in (gsum, vmin, psum, e, Gc))
+        return d
+
+    def identification_step():
+        nonlocal p, u, z, r, t, g_prev
+        pd = p.astype(np.float64)
+        f_cur = float((pd * (0.5 * u.astype(np.float64) + g0s.astype(np.float64))).sum())
+        if not (st["s_id"] > 0.0):
+            pp = float((pd * pd).sum())
+            pu = float((pd * u.astype(np.float64)).sum())
+            st["s_id"] = pp / pu if (pu > 0.0 and pp > 0.0) else 1.0
+        for trial in range(12):
+            s = R(st["s_id"])
+            z = (p - s * g).astype(dtype)
+            project_load_simplex(z, P_total)
+            r = apply(z)
+            zd = z.astype(np.float64)
+            f_try = float((zd * (0.5 * r.astype(np.float64) + g0s.astype(np.float64))).sum())
+            gd = float((g.astype(np.float64) * (z - p).astype(dtype).astype(np.float64)).sum())
+            if math.isfinite(f_try) and gd < 0.0 and f_try <= f_cur + 1e-4 * gd:
+                p, z = z, p
+                u, r = r, u
+                st["u_fresh"] = True
+                st["delta"] = R(0)
+                t = np.zeros(N, dtype=dtype)
+                g_prev = np.zeros(N, dtype=dtype)
+                st["G_old"] = 1.0
+                res.identification_steps += 1
+                if trial == 0:
+                    st["s_id"] *= 2.0
+                return True
+            st["s_id"] *= 0.25
+        return False
+
+    status = "max_iterations"
+    reason = ""
+    best_merit = math.inf
+    seen_merit = math.inf
+    stall = 0
+    forced_id = 0
+    stall_limit = opt.stall_limit if opt.stall_limit > 0 else 200
+    d = None
+    it = 0
+    while it < opt.max_iter:
+        if not st["u_fresh"]:
+            u = apply(p)
+        st["u_fresh"] = False
+        d = evaluate()
+        if not d.finite:
+            status = "nonfinite"
+            break
+        if opt.record_history:
+            res.error_history.append(d.pk)
+        if d.load <= load_tol and d.fw <= tol and d.pen <= tol:
+            status = "converged"
+            st["u_fresh"] = True
+            break
+        merit = max(d.fw, d.pen)
+        if opt.keep_best and merit < best_merit:
+            best_merit = merit
+            best_p = p.copy()
+        if merit < seen_merit * (1.0 - 1e-4):
+            seen_merit = merit
+            stall = 0
+        else:
+            stall += 1
+            if stall >= stall_limit:
+                if forced_id < 3 and identification_step():
+                    forced_id += 1
+                    stall = 0
+                    it += 1
+                    continue
+                status = "stagnated"
+                reason = "stall"
+                break
+        did_cg = False
+        if d.nc >= 2:
+            contact = p > 0
+            if precond is not None:
+                z = np.asarray(precond(g, contact), dtype=dtype).ravel()
+                cnt["pc"] += 1
+            else:
+                z = np.where(contact, g, R(0)).astype(dtype)
+            gd = g.astype(np.float64)[contact]
+            zd = z.astype(np.float64)[contact]
+            G = float((zd * gd).sum())
+            G_pr = float((zd * (g - g_prev).astype(dtype).astype(np.float64)[contact]).sum())
+            gg = float((gd * gd).sum())
+            G_old = st["G_old"]
+            beta_val = ((max(0.0, G_pr / G_old) if opt.use_pr else G / G_old)
+                        if G_old != 0.0 else 0.0)
+            beta = st["delta"] * R(beta_val)
+            t = np.where(contact, z + beta * t, R(0)).astype(dtype)
+            g_prev = g.copy()
+            st["G_old"] = G
+            r = apply(t)
+            rsum = float(r.astype(np.float64)[contact].sum())
+            num = float((g.astype(np.float64)[contact] * t.astype(np.float64)[contact]).sum())
+            rmean = R(rsum / d.nc if d.nc else 0.0)
+            den = float(((r - rmean).astype(dtype).astype(np.float64)[contact]
+                         * t.astype(np.float64)[contact]).sum())
+            if den > 0.0 and math.isfinite(den) and math.isfinite(num):
+                tau = R(num / den)
+                tau_ov = R(num * num / (den * gg)) if gg > 0.0 else tau
+                was = p > 0
+                pi = np.where(was, np.maximum(p - tau * t, R(0)), p).astype(dtype)
+                act = (pi == 0) & (g < 0)
+                pi = np.where(act, pi - tau_ov * g, pi).astype(dtype)
+                changed = bool(np.any((pi > 0) != was))
+                p = pi
+                st["delta"] = R(0) if changed else R(1)
+                total = float(p.astype(np.float64).sum())
+                if total > 0.0:
+                    p = (p * R(P_total / total)).astype(dtype)
+                else:
+                    p = np.full(N, p_bar, dtype=dtype)
+                did_cg = True
+        if not did_cg:
+            if not identification_step():
+                status = "nonpositive_curvature"
+                break
+        it += 1
+
+    if status != "converged" and opt.keep_best and best_p is not None:
+        merit = max(d.fw, d.pen)
+        if not (merit <= best_merit):
+            p = best_p
+            st["u_fresh"] = False
+            res.returned_best = True
+    best_p = None
+    if not st["u_fresh"]:
+        u = apply(p)
+    d = evaluate()
+    if not d.finite and status != "nonfinite":
+        status = "nonfinite"
+    if status == "converged" and not (d.load <= load_tol and d.fw <= tol and d.pen <= tol):
+        status = "verification_failed"
+        reason = "final_check"
+    if status == "converged" and req_tol < tol and not opt.allow_tolerance_relaxation:
+        if not (d.fw <= req_tol and d.pen <= req_tol):
+            status = "stagnated"
+            reason = "precision_limit"
+
+    if not opt.light:
+        res.displacement = u.astype(np.float64)
+        res.gap = g.astype(np.float64)
+    res.approach = d.alpha + datum
+    obj = float((p.astype(np.float64) * (0.5 * u.astype(np.float64)
+                                          + g0s.astype(np.float64))).sum())
+    res.objective = obj + datum * P_total
+    res.iterations = it
+    res.contact_fraction = d.nc / N
+    res.mean_pressure = d.psum / N
+    res.status = status
+    res.status_reason = reason
+    res.converged = status == "converged"
+    res.error = res.pk_error = d.pk
+    res.fw_gap = d.G
+    res.fw_error = d.fw
+    res.load_error = d.load
+    res.penetration_error = d.pen
+    res.requested_tol = req_tol
+    res.effective_tol = tol
+    res.p_ref = p_ref
+    res.g_ref = g_ref
+    res.matvec_count = cnt["mv"]
+    res.precond_count = cnt["pc"]
+    res.time_total = time.perf_counter() - t_start
+    res.pressure = p.astype(np.float64)
+    return res
+
+
+# ── L6: nested driver, fft / periodic branches (src/nested_solve.cpp) ───────
+
+def restrict_field(f: np.ndarray, Ns: int) -> np.ndarray:
+    """2x2 block average (src/nested_solve.cpp:42-54)."""
+    F = f.reshape(Ns, Ns)
+    return (0.25 * (F[0::2, 0::2] + F[0::2, 1::2] + F[1::2, 0::2] + F[1::2, 1::2])).ravel()
+
+
+def prolong_field(pc: np.ndarray, Nc: int) -> np.ndarray:
+    """Injection (src/nested_solve.cpp:57-71)."""
+    P = pc.reshape(Nc, Nc)
+    return np.repeat(np.repeat(P, 2, axis=0), 2, axis=1).ravel()
+
+
+def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
+                 precond_engine="auto", precond_radius=2, precond_occupancy_max=0.4,
+                 tol=1e-8, coarse_tol=1e-4, max_iter=20000, use_pr=True,
+                 precision="double", float_floor=2e-6,
+                 allow_tolerance_relaxation=False, light_result=False,
+                 backend="fft", record_error_history=False, leaf_side=8,
+                 operators: dict = None) -> Result:
+    """solve_contact_nested for backend fft|periodic (src/nested_solve.cpp:518-990).
+
+    `operators` may map level size -> prebuilt operator (to reuse tables)."""
+    g0 = np.ascontiguousarray(g0, dtype=np.float64).ravel()
+    if g0.size != Ns * Ns:
+        raise ValueError("g0 size != Ns*Ns")
+    levels = []
+    n = coarsest
+    while n <= Ns:
+        levels.append(n)
+        n *= 2
+    if not levels or levels[-1] != Ns:
+        raise ValueError("Ns must equal coarsest * 2^k")
+    any_float = precision != "double"
+    ncoarse = len(levels) - 1
+    gap = [None] * ncoarse
+    if ncoarse > 0:
+        gap[ncoarse - 1] = restrict_field(g0, Ns)
+        for i in range(ncoarse - 2, -1, -1):
+            gap[i] = restrict_field(gap[i + 1], levels[i + 1])
+    gmn, gmx = float(g0.min()), float(g0.max())
+    datum = gmn + 0.5 * (gmx - gmn)
+    g_ref = max(gmx - gmn, p_bar * L / E_star)
+    p_init = None
+    prev_occ = -1.0
+    stages = []
+    res = None
+    t0 = time.perf_counter()
+    for li, n in enumerate(levels):
+        finest = li + 1 == len(levels)
+        if operators and n in operators:
+            op = operators[n]
+        elif backend == "fft":
+            op = FreeSpaceOperator(n, L, E_star)
+        else:
+            op = PeriodicOperator(n, L, E_star)
+        r_lvl = min(precond_radius, max(1, n // 4), leaf_side)
+        engine = "fft" if (backend == "periodic" and precond_engine == "auto") else precond_engine
+        use_stencil = stencil_for_level(engine, precond, prev_occ, precond_occupancy_max)
+        pc_obj = None
+        if use_stencil:
+            pc_obj = StencilPreconditioner(n, r_lvl)
+        elif precond:
+            pc_obj = FourierPreconditioner(n)
+        glvl = g0 if finest else gap[li]
+        polish_after = finest and precision == "float_then_double"
+        lvl_tol = tol if finest else coarse_tol
+        if polish_after:
+            lvl_tol = float_floor
+        elif any_float:
+            lvl_tol = max(lvl_tol, float_floor)
+        light = light_result if finest else True
+        lopt = SolveOptions(tol=lvl_tol, requested_tol=(tol if finest else coarse_tol),
+                            max_iter=max_iter, use_pr=use_pr, light=light,
+                            record_history=finest and record_error_history,
+                            keep_best=not light,
+                            allow_tolerance_relaxation=(not finest) or polish_after
+                            or allow_tolerance_relaxation,
+                            g_ref=g_ref, datum=datum,
+                            datum_mode=("caller" if any_float else "solver"))
+        dt = np.float32 if any_float else np.float64
+        pcf = (lambda gg, cc, _o=pc_obj, _d=dt: _o.apply(gg, cc, _d)) if pc_obj else None
+        mvf = lambda x, _o=op, _d=dt: _o.matvec(x, _d)
+        if any_float:
+            gf = (glvl - datum).astype(np.float32)
+            res = solve_contact(mvf, gf, np.float32(p_bar), lopt, pcf,
+                                None if p_init is None else p_init.astype(np.float32),
+                                dtype=np.float32)
+        else:
+            res = solve_contact(mvf, glvl, p_bar, lopt, pcf, p_init, dtype=np.float64)
+        prev_occ = res.contact_fraction
+        stages.append(dict(name=("finest:" if finest else "coarse:") + str(n),
+                           precision="float" if any_float else "double",
+                           requested_tol=lopt.requested_tol, effective_tol=lopt.tol,
+                           iterations=res.iterations, matvec_count=res.matvec_count,
+                           status=res.status, fw_error=res.fw_error,
+                           penetration_error=res.penetration_error))
+        if polish_after:
+            popt = SolveOptions(tol=tol, requested_tol=tol, max_iter=max_iter,
+                                use_pr=use_pr, light=light_result,
+                                record_history=finest and record_error_history,
+                                keep_best=not light_result,
+                                allow_tolerance_relaxation=allow_tolerance_relaxation,
+                                g_ref=g_ref, datum=datum, datum_mode="solver")
+            before = res
+            pcd = (lambda gg, cc, _o=pc_obj: _o.apply(gg, cc, np.float64)) if pc_obj else None
+            res = solve_contact(lambda x, _o=op: _o.matvec(x, np.float64), glvl, p_bar,
+                                popt, pcd, before.pressure, dtype=np.float64)
+            stages.append(dict(name="polish:" + str(n), precision="double",
+                               requested_tol=tol, effective_tol=tol,
+                               iterations=res.iterations, matvec_count=res.matvec_count,
+                               status=res.status, fw_error=res.fw_error,
+                               penetration_error=res.penetration_error))
+            res.iterations += before.iterations
+            res.matvec_count += before.matvec_count
+            res.precond_count += before.precond_count
+            if record_error_history:
+                res.error_history = list(before.error_history) + list(res.error_history)
+        if not finest:
+            p_init = prolong_field(res.pressure, n)
+    res.stage_stats = stages
+    res.time_total = time.perf_counter() - t0
+    return res
+
+
+# ── independent diagnostics (tests/contact_oracle.hpp) ──────────────────────
+
+def diagnose(matvec: Callable, g0, p, P_total, g_ref) -> dict:
+    """tests/contact_oracle.hpp:35-61, with an arbitrary FP64 operator."""
+    p = np.asarray(p, dtype=np.float64).ravel()
+    v = np.asarray(matvec(p), dtype=np.float64) + np.asarray(g0, dtype=np.float64).ravel()
+    pos = p > 0
+    nc = int(pos.sum())
+    approach = v[pos].sum() / nc if nc else v.min()
+    min_gap = v.min() - approach
+    G = float((p * (v - v.min())).sum())
+    return dict(approach=approach, min_gap=min_gap, fw_gap=G,
+                fw_error=G / (P_total * g_ref),
+                penetration=max(0.0, -min_gap) / g_ref,
+                load_error=abs(p.sum() - P_total) / P_total,
+                min_pressure=float(p.min()),
+                pk_error=float((p * np.abs(v - approach)).sum()) / (P_total * g_ref),
+                objective=0.5 * float(p @ (v - np.asarray(g0).ravel())) + float(p @ np.asarray(g0).ravel()),
+                n_contact=nc)
+
+
+def sorting_projection(y, P_total):
+    """tests/contact_oracle.hpp:64-80."""
+    u = np.sort(np.asarray(y, dtype=np.float64))[::-1]
+    css = np.cumsum(u)
+    t = (css - P_total) / np.arange(1, u.size + 1)
+    ok = u > t
+    theta = t[np.nonzero(ok)[0][-1]] if ok.any() else 0.0
+    return np.maximum(np.asarray(y, dtype=np.float64) - theta, 0.0)
+
+
+def enumerate_qp(S, g0, P_total, tol=1e-10):
+    """Support enumeration, n <= 12 (tests/contact_oracle.hpp:95-142)."""
+    n = g0.size
+    best = (math.inf, None, 0.0)
+    for m in range(1, 1 << n):
+        A = [i for i in range(n) if m & (1 << i)]
+        k = len(A)
+        K = np.zeros((k + 1, k + 1))
+        rhs = np.zeros(k + 1)
+        K[:k, :k] = S[np.ix_(A, A)]
+        K[:k, k] = -1.0
+        K[k, :k] = 1.0
+        rhs[:k] = -g0[A]
+        rhs[k] = P_total
+        try:
+            sol = np.linalg.solve(K, rhs)
+        except np.linalg.LinAlgError:
+            continue
+        if np.linalg.norm(K @ sol - rhs) > 1e-8 * (1.0 + np.linalg.norm(rhs)):
+            continue
+        if np.any(sol[:k] < -tol):
+            continue
+        p = np.zeros(n)
+        p[A] = np.maximum(sol[:k], 0.0)
+        alpha = sol[k]
+        v = S @ p + g0
+        if np.any(v - alpha < -tol * (1.0 + abs(alpha))):
+            continue
+        obj = 0.5 * p @ (S @ p) + p @ g0
+        if obj < best[0]:
+            best = (obj, p, alpha)
+    return best
+
+
+def manufacture(S, p_star, g_star, alpha_star):
+    """Manufactured KKT fixture (tests/contact_oracle.hpp:156-166)."""
+    g0 = alpha_star - S @ p_star + g_star
+    return g0, float(p_star.sum())
+
+
+def spd_kernel(n, decay=1.0):
+    """tests/contact_oracle.hpp:169-177."""
+    i = np.arange(n)
+    d = np.abs(i[:, None] - i[None, :]).astype(np.float64)
+    S = np.exp(-decay * d) / (1.0 + 0.25 * d) + 0.1 * np.eye(n)
+    return 0.5 * (S + S.T)

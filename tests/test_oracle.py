@@ -1,0 +1,391 @@
+"""Pins the CPU oracle (oracle/) against the reference's own fixtures, closed
+forms and mathematical invariants. CPU only — no GPU, no product code.
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails one of them.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _love_fixture():
+    rows = []
+    with open(os.path.join(GOLD, "love_reference.txt")) as f:
+        for line in f:
+            if not line.strip() or line.startswith("#"):
+                continue
+            rows.append([float(v) for v in line.split()])
+    return np.array(rows)
+
+
+# ── a1: love_uz vs the reference's 80-digit fixture ─────────────────────────
+
+def test_love_fixture_1e12():
+    """tests/test_kernel_stability.cpp:53-85: 472 cases, rel <= 1e-12."""
+    F = _love_fixture()
+    assert F.shape[0] > 300
+    got = O.love_uz(F[:, 0], F[:, 1], F[:, 2], F[:, 3])
+    rel = np.abs(got - F[:, 4]) / np.abs(F[:, 4])
+    assert np.all(np.isfinite(got)) and np.all(got > 0)
+    assert rel.max() <= 1e-12, rel.max()
+
+
+def test_love_self_term_parity_homogeneity():
+    """tests/test_kernel_stability.cpp:88-100,150-185."""
+    self_t = O.love_uz(0.0, 0.0, 0.5, 0.5)[0]
+    assert abs(self_t - 4 * math.log(1 + math.sqrt(2))) <= 1e-15 * self_t
+    for x in (0.0, 0.3, 1.0, 7.5, 1234.0, 5e5):
+        for y in (0.0, 0.2, 2.0, 33.0, 9999.0, 3e5):
+            v = O.love_uz(x, y, 0.5, 0.5)[0]
+            assert O.love_uz(-x, y, 0.5, 0.5)[0] == v
+            assert O.love_uz(x, -y, 0.5, 0.5)[0] == v
+            sw = O.love_uz(y, x, 0.5, 0.5)[0]
+            assert abs(sw - v) <= 1e-12 * v
+    for s in (1e-3, 0.1, 3.0, 1e3, 1e6):
+        for x in (0.0, 0.7, 12.0, 4000.0):
+            y = 0.31 * x + 0.05
+            v = O.love_uz(x, y, 0.5, 0.4)[0]
+            vs = O.love_uz(s * x, s * y, s * 0.5, s * 0.4)[0]
+            assert abs(vs - s * v) <= 1e-13 * abs(s * v)
+
+
+def test_love_table_kernel_checks():
+    """tests/test_kernel.cpp:15-56 (self term, point-source asymptotics, decay)."""
+    Ns, L, E = 64, 1.0, 1.7
+    h = L / Ns
+    T = O.love_table(Ns, L, E)
+    exact = 4 * h * math.log(1 + math.sqrt(2)) / (math.pi * E)
+    assert abs(T[0, 0] / exact - 1) < 1e-14
+
+    def rel(dx, dy):
+        r = h * math.hypot(dx, dy)
+        return abs(T[dy, dx] / (h * h / (math.pi * E * r)) - 1)
+    assert 0.03 < rel(1, 0) < 0.05
+    assert 0.015 < rel(1, 1) < 0.035
+    assert rel(5, 0) < 0.002 and rel(4, 3) < 0.002 and rel(20, 11) < 0.0005
+    assert np.all(np.diff(T[0, :]) < 0) and np.all(T > 0)
+
+
+# ── a3/a5: free-space operator vs brute-force dense ─────────────────────────
+
+@pytest.mark.parametrize("Ns", [8, 24, 32])
+def test_free_space_matches_dense(Ns):
+    """tests/test_fft.cpp:20-39: double < 1e-12, float < 1e-5."""
+    op = O.FreeSpaceOperator(Ns)
+    S = O.dense_matrix(Ns)
+    x = np.random.default_rng(Ns).uniform(-1, 1, Ns * Ns)
+    ref = S @ x
+    y = op.matvec(x)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-12
+    yf = op.matvec(x.astype(np.float32), np.float32)
+    assert np.linalg.norm(yf - ref) / np.linalg.norm(ref) < 1e-5
+    # the kernel spectrum is real and even in ky (only (Ns+1)^2 unique values)
+    M = 2 * Ns
+    assert np.allclose(op.Kh[1:, :], op.Kh[1:, :][::-1, :], rtol=0, atol=1e-13 * np.abs(op.Kh).max())
+    assert op.Kh.shape == (M, Ns + 1)
+
+
+def test_free_space_dense_symmetric():
+    S = O.dense_matrix(6)
+    assert np.array_equal(S, S.T)
+    assert np.linalg.eigvalsh(S).min() > 0
+
+
+# ── a6: periodic operator closed forms ──────────────────────────────────────
+
+def test_periodic_cos_modes():
+    """tests/test_periodic.cpp:16-36 (< 2e-13)."""
+    n, L, E = 32, 3.0, 2.5
+    op = O.PeriodicOperator(n, L, E)
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    for mx, my in ((1, 0), (0, 3), (2, 5)):
+        q = 2 * math.pi * math.hypot(mx, my) / L
+        p = np.cos(2 * math.pi * (mx * ix + my * iy) / n).ravel()
+        u = op.matvec(p)
+        ref = 2.0 / (E * q) * p
+        assert np.linalg.norm(u - ref) / np.linalg.norm(ref) < 2e-13
+
+
+@pytest.mark.parametrize("n", [7, 12, 16])
+def test_periodic_matrix_properties(n):
+    """tests/test_periodic.cpp:56-88: symmetric, PSD, circulant, DC null space."""
+    op = O.PeriodicOperator(n, 1.7, 0.8)
+    N = n * n
+    S = np.stack([op.matvec(np.eye(N)[j]) for j in range(N)], axis=1)
+    assert np.linalg.norm(S - S.T) / np.linalg.norm(S) < 1e-14
+    ev = np.linalg.eigvalsh(S)
+    assert ev[0] > -1e-13 * ev[-1] and ev[1] > 1e-6 * ev[-1]
+    assert np.linalg.norm(op.matvec(np.ones(N))) < 1e-13
+    sx, sy = 3 % n, 5 % n
+    col = S[:, sy * n + sx].reshape(n, n)
+    col0 = S[:, 0].reshape(n, n)
+    assert np.allclose(col, np.roll(np.roll(col0, sy, 0), sx, 1), rtol=0, atol=1e-14 * S[0, 0])
+
+
+# ── a7/a8: preconditioners ──────────────────────────────────────────────────
+
+@pytest.mark.parametrize("Ns", [512, 1024])
+def test_stencil_taps_match_reference_table(Ns):
+    """tests/test_stencil.cpp:36-58: the universal-constant oracle table."""
+    fx = json.load(open(os.path.join(GOLD, "stencil_taps.json")))
+    sp = O.StencilPreconditioner(Ns, 2)
+    assert len(sp.w) == 13
+    for dx, dy, w in zip(sp.dx, sp.dy, sp.w):
+        a, b = sorted((abs(dx), abs(dy)), reverse=True)
+        ref = fx["taps"]["%d,%d" % (a, b)]
+        assert abs(w - ref) <= fx["tolerance"], (dx, dy, w, ref)
+        if dx == 0 and dy == 0:
+            assert w == 1.0
+
+
+@pytest.mark.parametrize("R", [1, 2, 4])
+def test_stencil_symbol_positive(R):
+    """tests/test_stencil.cpp:61-84."""
+    Ns = 64
+    sp = O.StencilPreconditioner(Ns, R)
+    k = np.arange(Ns)
+    KX, KY = np.meshgrid(k, k, indexing="xy")
+    s = np.zeros((Ns, Ns))
+    for dx, dy, w in zip(sp.dx, sp.dy, sp.w):
+        s += w * np.cos(2 * math.pi * (KX * dx + KY * dy) / Ns)
+    s[0, 0] = 1.0
+    assert s.min() > 0
+
+
+def test_preconditioners_offset_invariant_and_symmetric():
+    """tests/test_precond.cpp:27-66."""
+    Ns = 32
+    N = Ns * Ns
+    rng = np.random.default_rng(7)
+    contact = rng.random(N) < 0.4
+    chi = contact.astype(np.float64)
+    g, x, y = (rng.uniform(-1, 1, N) for _ in range(3))
+    for pc in (O.FourierPreconditioner(Ns), O.StencilPreconditioner(Ns, 2)):
+        z0 = pc.apply(g, contact)
+        z1 = pc.apply(g + 0.37 * chi, contact)
+        assert np.abs(z1 - z0).max() / np.abs(z0).max() < 1e-12
+        mx, my = pc.apply(x, contact), pc.apply(y, contact)
+        xmy = (x * my)[contact].sum()
+        ymx = (y * mx)[contact].sum()
+        assert abs(xmy - ymx) / (abs(xmy) + abs(ymx)) < 1e-12
+        assert np.all(z0[~contact] == 0)
+        assert abs(z0[contact].sum()) < 1e-12 * np.abs(z0).sum()
+
+
+def test_stencil_for_level_table():
+    """tests/test_precond.cpp:68-101."""
+    f = O.stencil_for_level
+    assert not f("stencil", False, -1, 0.4) and not f("auto", False, 0.9, 0.4)
+    assert f("stencil", True, 0.9, 0.4) and not f("fft", True, -1, 0.4)
+    assert f("auto", True, -1, 0.4) and f("auto", True, 0.39, 0.4)
+    assert not f("auto", True, 0.4, 0.4) and not f("auto", True, 0.41, 0.4)
+    assert f("auto", True, 0.55, 0.6) and not f("auto", True, 0.65, 0.6)
+
+
+# ── a9: projection vs sorting oracle ────────────────────────────────────────
+
+def test_projection_matches_sorting():
+    """tests/test_certification.cpp:443-467 (T21)."""
+    rng = np.random.default_rng(7)
+    for trial in range(200):
+        n = 1 + trial % 40
+        scale = 10.0 ** ((trial % 13) - 6)
+        y = scale * rng.uniform(-1, 1, n)
+        if trial % 5 == 0:
+            y[:] = scale
+        P = 0.37 * scale * n
+        prod = y.copy()
+        O.project_load_simplex(prod, P)
+        ref = O.sorting_projection(y, P)
+        assert prod.min() >= 0
+        assert abs(prod.sum() - P) <= 1e-12 * P * max(1, n)
+        assert np.linalg.norm(prod - ref) / max(np.linalg.norm(ref), 1e-300) < 1e-12
+        again = prod.copy()
+        O.project_load_simplex(again, P)
+        assert np.linalg.norm(again - prod) <= 1e-12 * max(np.linalg.norm(prod), 1e-300)
+
+
+# ── a10–a13: the PK solver against exact solutions ──────────────────────────
+
+def _dense_op(S):
+    return lambda v: S @ v
+
+
+@pytest.mark.parametrize("case", ["singleton", "full", "islands"])
+def test_manufactured_kkt(case):
+    """T01 (tests/test_certification.cpp:76-152): known p*, enumeration oracle."""
+    n = 10
+    S = O.spd_kernel(n)
+    p = np.zeros(n)
+    g = np.zeros(n)
+    if case == "singleton":
+        p[4] = 1.0
+        g = np.array([0.0 if i == 4 else 0.5 + 0.1 * i for i in range(n)])
+    elif case == "full":
+        p = 0.3 + 0.05 * np.arange(n)
+    else:
+        p[[0, 1, 7, 8]] = [0.7, 0.4, 0.9, 0.2]
+        for i in range(n):
+            if p[i] == 0.0:
+                g[i] = 0.0 if i == 5 else 0.3 + 0.2 * i
+    g0, P = O.manufacture(S, p, g, 0.25)
+    gref = g0.max() - g0.min()
+    obj_star = 0.5 * p @ S @ p + p @ g0
+    en = O.enumerate_qp(S, g0, P)
+    assert en[1] is not None and abs(en[0] - obj_star) < 1e-9 * (1 + abs(obj_star))
+    for warm in range(3):
+        p0 = None
+        if warm == 1:
+            p0 = p.copy()
+        if warm == 2:
+            p0 = np.zeros(n)
+            p0[-1] = P
+        opt = O.SolveOptions(tol=1e-12, max_iter=20000, g_ref=gref)
+        r = O.solve_contact(_dense_op(S), g0, P / n, opt, None, p0)
+        d = O.diagnose(_dense_op(S), g0, r.pressure, P, gref)
+        assert r.status == "converged"
+        assert d["fw_error"] <= 1e-12 and d["penetration"] <= 1e-12
+        assert d["load_error"] <= 1e-12 and d["min_pressure"] >= 0
+        assert np.linalg.norm(r.pressure - p) / np.linalg.norm(p) < 1e-7
+        assert abs(r.fw_error - d["fw_error"]) <= 1e-9 * max(1, d["fw_error"])
+        assert abs(r.approach - d["approach"]) <= 1e-9 * max(1, abs(d["approach"]))
+        if warm == 1:
+            assert r.iterations == 0
+        if warm == 2:
+            assert r.iterations > 0
+
+
+def test_support_enumeration_random():
+    """Brute force on tiny Love problems: PK == enumeration."""
+    Ns = 3
+    S = O.dense_matrix(Ns)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        g0 = rng.uniform(0, 0.05, Ns * Ns)
+        P = 0.01 * Ns * Ns
+        en = O.enumerate_qp(S, g0, P)
+        r = O.solve_contact(_dense_op(S), g0, 0.01, O.SolveOptions(tol=1e-13, max_iter=5000))
+        assert r.status == "converged"
+        assert np.linalg.norm(r.pressure - en[1]) / np.linalg.norm(en[1]) < 1e-8
+
+
+def test_status_paths():
+    """T03 (tests/test_certification.cpp:215-323)."""
+    Ns = 16
+    S = O.dense_matrix(Ns)
+    x = (np.arange(Ns) + 0.5) / Ns - 0.5
+    g0 = (x[:, None] ** 2 + x[None, :] ** 2).ravel()
+    for mi in (0, 1):
+        r = O.solve_contact(_dense_op(S), g0, 0.02, O.SolveOptions(tol=1e-10, max_iter=mi))
+        assert r.status == "max_iterations" and not r.converged
+    calls = [0]
+
+    def bad(v):
+        calls[0] += 1
+        y = S @ v
+        if calls[0] > 3:
+            y[0] = np.nan
+        return y
+    r = O.solve_contact(bad, g0, 0.02, O.SolveOptions(tol=1e-10, max_iter=100))
+    assert r.status == "nonfinite"
+    gb = g0.copy()
+    gb[3] = np.inf
+    with pytest.raises(ValueError):
+        O.solve_contact(_dense_op(S), gb, 0.02)
+    for args in ((-1.0, 1e-10, 10), (0.02, 0.0, 10), (0.02, 1e-10, -3)):
+        with pytest.raises(ValueError):
+            O.solve_contact(_dense_op(S), g0, args[0],
+                            O.SolveOptions(tol=args[1], max_iter=args[2]))
+
+
+def test_fw_bound_identities():
+    """T31 (tests/test_certification.cpp:376-440): 0 <= f-f* <= G, |p-p*|_S^2 <= 2G."""
+    n = 8
+    S = O.spd_kernel(n, 0.7)
+    p_star = np.zeros(n)
+    p_star[[1, 2, 6]] = [0.5, 1.5, 0.75]
+    g_star = np.where(p_star > 0, 0.0, 0.2 + 0.05 * np.arange(n))
+    g0, P = O.manufacture(S, p_star, g_star, 0.1)
+    gref = g0.max() - g0.min()
+    f_star = 0.5 * p_star @ S @ p_star + p_star @ g0
+    rng = np.random.default_rng(12345)
+    for _ in range(200):
+        p = O.sorting_projection(rng.random(n), P)
+        d = O.diagnose(_dense_op(S), g0, p, P, gref)
+        gap = d["objective"] - f_star
+        assert gap >= -1e-12 and gap <= d["fw_gap"] + 1e-12
+        e = p - p_star
+        assert e @ S @ e <= 2 * d["fw_gap"] + 1e-12
+
+
+def test_westergaard_full_and_partial():
+    """tests/test_periodic.cpp:94-129 (n reduced 256 -> 128 for CPU time)."""
+    n, L, Es, D = 128, 2.0, 1.3, 0.01
+    pstar = math.pi * Es * D / L
+    op = O.PeriodicOperator(n, L, Es)
+    cosx = np.tile(np.cos(2 * math.pi * np.arange(n) / n), (n, 1)).ravel()
+    g0 = D * (1 - cosx)
+    pbar = 1.5 * pstar
+    r = O.solve_contact(op.matvec, g0, pbar, O.SolveOptions(tol=1e-12))
+    assert r.status == "converged"
+    pex = pbar + pstar * cosx
+    assert np.linalg.norm(r.pressure - pex) / np.linalg.norm(pex) < 1e-9
+    pbar = 0.3 * pstar
+    r = O.solve_contact(op.matvec, g0, pbar, O.SolveOptions(tol=1e-10))
+    assert r.status == "converged"
+    width = (r.pressure[:n] > 0).sum() * L / n
+    wex = 2 * (L / math.pi) * math.asin(math.sqrt(pbar / pstar))
+    assert abs(width - wex) <= 2 * L / n
+
+
+def test_hertz_fft_equals_dense():
+    """tests/test_fft.cpp:63-107 (Ns 64 -> 32): fft vs dense solve."""
+    Ns, R, pbar = 32, 0.5, 0.05
+    h = 1.0 / Ns
+    c = (np.arange(Ns) + 0.5) * h - 0.5
+    g0 = ((c[:, None] ** 2 + c[None, :] ** 2) / (2 * R)).ravel()
+    S = O.dense_matrix(Ns)
+    op = O.FreeSpaceOperator(Ns)
+    rd = O.solve_contact(_dense_op(S), g0, pbar, O.SolveOptions(tol=1e-8, max_iter=2000))
+    rf = O.solve_contact(op.matvec, g0, pbar, O.SolveOptions(tol=1e-8, max_iter=2000))
+    assert rd.converged and rf.converged
+    assert abs(rd.iterations - rf.iterations) <= 1
+    assert rd.contact_fraction == rf.contact_fraction
+    assert np.linalg.norm(rf.pressure - rd.pressure) / np.linalg.norm(rd.pressure) < 1e-6
+
+
+def test_nested_matches_single_level_and_datum():
+    """tests/test_periodic_py.py:53-67 and tests/test_certification_py.py:96-121."""
+    ns = 64
+    rng = np.random.default_rng(3)
+    k = np.fft.fftfreq(ns, 1.0 / ns)
+    kk = np.hypot(k[:, None], k[None, :])
+    kk[0, 0] = 1
+    amp = np.where((kk >= 2) & (kk <= ns // 8), kk ** -1.8, 0.0)
+    h = np.real(np.fft.ifft2(amp * np.exp(2j * np.pi * rng.random((ns, ns)))))
+    h *= 0.01 / h.std()
+    g = (h.max() - h).ravel()
+    op = O.PeriodicOperator(ns)
+    fp = O.FourierPreconditioner(ns)
+    ref = O.solve_contact(op.matvec, g, 0.01, O.SolveOptions(tol=1e-12), fp.apply)
+    for prec, tol, bound in (("double", 1e-10, 1e-7), ("float_then_double", 1e-10, 1e-7),
+                             ("float", 2e-6, 1e-4)):
+        r = O.solve_nested(ns, g, 0.01, coarsest=16, backend="periodic", tol=tol,
+                           precision=prec)
+        assert r.status == "converged", (prec, r.status, r.status_reason)
+        assert abs(r.pressure.mean() - 0.01) < 1e-6 * 0.01
+        assert np.linalg.norm(r.pressure - ref.pressure) / np.linalg.norm(ref.pressure) < bound
+    base = O.solve_nested(ns, g, 0.01, coarsest=16, backend="fft", tol=1e-10)
+    for shift in (1e3, 1e6):
+        r = O.solve_nested(ns, g + shift, 0.01, coarsest=16, backend="fft", tol=1e-10)
+        assert r.status == "converged"
+        assert abs(r.approach - base.approach - shift) < 1e-6 * shift
+        assert np.linalg.norm(r.pressure - base.pressure) / np.linalg.norm(base.pressure) < 1e-5
