@@ -17,13 +17,15 @@ OK, ERR_INVALID, ERR_LOGIC, ERR_RUNTIME, ERR_DEVICE = 0, 1, 2, 3, 4
 EXPORTS = (
     "aspher_last_error", "aspher_version", "aspher_device_count",
     "aspher_solver_create", "aspher_solver_destroy", "aspher_solver_matvec",
-    "aspher_solver_matvec_device", "aspher_solver_time_matvec", "aspher_solver_solve",
+    "aspher_solver_matvec_device", "aspher_solver_time_matvec", "aspher_solver_time_stages",
+    "aspher_solver_solve",
     "aspher_solver_solve_device", "aspher_solver_info", "aspher_nested_params_default",
     "aspher_solve_nested", "aspher_solve_nested_device", "aspher_result_get_info",
     "aspher_result_get_field", "aspher_result_field_device", "aspher_result_get_history",
     "aspher_result_get_stage", "aspher_result_destroy", "aspher_configure_allocator",
     "aspher_love_points", "aspher_love_table", "aspher_stencil_taps",
-    "aspher_precond_apply", "aspher_clear_cache",
+    "aspher_precond_apply", "aspher_clear_cache", "aspher_device_malloc",
+    "aspher_device_free", "aspher_memcpy_htod", "aspher_memcpy_dtoh",
 )
 
 
@@ -95,6 +97,7 @@ def _declare(L):
         "aspher_solver_matvec": (i, [vp, dp, dp]),
         "aspher_solver_matvec_device": (i, [vp, vp, vp, i]),
         "aspher_solver_time_matvec": (i, [vp, vp, vp, i, i, dp]),
+        "aspher_solver_time_stages": (i, [vp, vp, vp, i, i, dp]),
         "aspher_solver_solve": (i, [vp, dp, d, d, i, i, s, dp, C.POINTER(vp)]),
         "aspher_solver_solve_device": (i, [vp, vp, d, d, i, i, s, vp, C.POINTER(vp)]),
         "aspher_solver_info": (i, [vp, C.POINTER(OperatorInfo)]),
@@ -115,6 +118,10 @@ def _declare(L):
         "aspher_stencil_taps": (i, [i, i, ip, ip, ip, dp]),
         "aspher_precond_apply": (i, [i, s, i, i, dp, C.POINTER(C.c_uint8), dp]),
         "aspher_clear_cache": (i, []),
+        "aspher_device_malloc": (i, [C.c_int64, C.POINTER(vp)]),
+        "aspher_device_free": (i, [vp]),
+        "aspher_memcpy_htod": (i, [vp, vp, C.c_int64]),
+        "aspher_memcpy_dtoh": (i, [vp, vp, C.c_int64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -194,6 +194,14 @@ class ContactSolver:
                                               int(precision), int(reps), C.byref(ms)))
         return ms.value
 
+    def time_stages(self, x_ptr: int, y_ptr: int, precision: int = 64, reps: int = 20):
+        """Average device milliseconds of the row-forward, column and
+        row-inverse launches of one matvec."""
+        ms = (C.c_double * 3)()
+        check(lib().aspher_solver_time_stages(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                              int(precision), int(reps), ms))
+        return [ms[0], ms[1], ms[2]]
+
     def solve(self, gap, p_nominal, tol=1e-8, max_iter=5000, use_pr=True, precond="none",
               p_init=None) -> ContactResult:
         """python/bindings.cpp:186-234 (defaults :818-828)."""

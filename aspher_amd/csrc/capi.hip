@@ -229,6 +229,39 @@ int aspher_solver_time_matvec(aspher_solver* s, const void* x, void* y, int prec
     });
 }
 
+int aspher_solver_time_stages(aspher_solver* s, const void* x, void* y, int precision, int reps,
+                              double* ms3) {
+    return guard([&] {
+        if (!s || !x || !y || !ms3 || reps <= 0) throw std::invalid_argument("bad arguments");
+        if (precision != 64 && precision != 32)
+            throw std::invalid_argument("precision must be 64 or 32");
+        ASPHER_HIP(hipDeviceSynchronize());
+        hipStream_t st = s->ctx->stream;
+        if (precision == 32) s->op->build_single();
+        std::vector<hipEvent_t> ev(4 * (std::size_t)reps);
+        for (auto& e : ev) ASPHER_HIP(hipEventCreate(&e));
+        auto run = [&](hipEvent_t* e) {
+            if (precision == 64)
+                s->op->apply_timed(static_cast<const double*>(x), static_cast<double*>(y), st, e);
+            else
+                s->op->apply_timed(static_cast<const float*>(x), static_cast<float*>(y), st, e);
+        };
+        run(ev.data());  // warm-up
+        ASPHER_HIP(hipStreamSynchronize(st));
+        for (int r = 0; r < reps; ++r) run(ev.data() + 4 * r);
+        ASPHER_HIP(hipStreamSynchronize(st));
+        double acc[3] = {0, 0, 0};
+        for (int r = 0; r < reps; ++r)
+            for (int k = 0; k < 3; ++k) {
+                float t = 0.0f;
+                ASPHER_HIP(hipEventElapsedTime(&t, ev[4 * r + k], ev[4 * r + k + 1]));
+                acc[k] += t;
+            }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < 3; ++k) ms3[k] = acc[k] / reps;
+    });
+}
+
 int aspher_solver_solve(aspher_solver* s, const double* gap, double p_nominal, double tol,
                         int max_iter, int use_pr, const char* precond, const double* p_init,
                         aspher_result** out) {
@@ -545,6 +578,33 @@ int aspher_precond_apply(int grid_size, const char* kind, int radius, int precis
 
 int aspher_clear_cache(void) {
     return guard([&] { clear_operator_cache(); });
+}
+
+int aspher_device_malloc(int64_t bytes, void** ptr) {
+    return guard([&] {
+        if (!ptr || bytes <= 0) throw std::invalid_argument("bad allocation request");
+        ASPHER_HIP(hipMalloc(ptr, (std::size_t)bytes));
+    });
+}
+
+int aspher_device_free(void* ptr) {
+    return guard([&] {
+        if (ptr) ASPHER_HIP(hipFree(ptr));
+    });
+}
+
+int aspher_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        if (!dst || !src || bytes < 0) throw std::invalid_argument("bad copy");
+        ASPHER_HIP(hipMemcpy(dst, src, (std::size_t)bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int aspher_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        if (!dst || !src || bytes < 0) throw std::invalid_argument("bad copy");
+        ASPHER_HIP(hipMemcpy(dst, src, (std::size_t)bytes, hipMemcpyDeviceToHost));
+    });
 }
 
 } // extern "C"

@@ -155,9 +155,8 @@ SpectralPlan<float>& Operator::plan<float>() {
     if (!plan32_) build_single();
     return *plan32_;
 }
-
 template <typename T>
-void Operator::apply_t(const T* x, T* y, hipStream_t s) {
+void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
     if (!built_) throw std::logic_error("FFTOperator::matvec: build() not called");
     if (kind_ == Kind::free_space) {
         const T* kh;
@@ -166,14 +165,17 @@ void Operator::apply_t(const T* x, T* y, hipStream_t s) {
             if (!kh32_.p) build_single();
             kh = kh32_.p;
         }
-        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::table, kh, Ns_ + 1, 0.0, s);
+        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::table, kh, Ns_ + 1, 0.0, s, RowIn::plain,
+                           RowOut::plain, nullptr, nullptr, 0, nullptr, ev);
     } else if (kind_ == Kind::periodic) {
         // q = 2 pi k/L, u_hat = 2 p_hat/(E*|q|), unnormalised pair -> 1/Ns^2
         // (src/periodic_fft_operator.cpp:11-24)
         const double pi = std::acos(-1.0);
         const double sc = L_ / (pi * E_ * static_cast<double>(Ns_) * Ns_);
-        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s);
+        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s, RowIn::plain,
+                           RowOut::plain, nullptr, nullptr, 0, nullptr, ev);
     } else {
+        if (ev) throw std::invalid_argument("stage timing needs a spectral backend");
         const std::int64_t N = (std::int64_t)Ns_ * Ns_;
         if (x == y) {
             DevBuf<T>& tmp = [&]() -> DevBuf<T>& {
@@ -189,8 +191,14 @@ void Operator::apply_t(const T* x, T* y, hipStream_t s) {
     }
 }
 
-void Operator::apply(const double* x, double* y, hipStream_t s) { apply_t<double>(x, y, s); }
-void Operator::apply(const float* x, float* y, hipStream_t s) { apply_t<float>(x, y, s); }
+void Operator::apply(const double* x, double* y, hipStream_t s) { apply_t<double>(x, y, s, nullptr); }
+void Operator::apply(const float* x, float* y, hipStream_t s) { apply_t<float>(x, y, s, nullptr); }
+void Operator::apply_timed(const double* x, double* y, hipStream_t s, hipEvent_t* ev) {
+    apply_t<double>(x, y, s, ev);
+}
+void Operator::apply_timed(const float* x, float* y, hipStream_t s, hipEvent_t* ev) {
+    apply_t<float>(x, y, s, ev);
+}
 
 std::size_t Operator::bytes_spectrum() const {
     return kh64_.bytes() + kh32_.bytes();

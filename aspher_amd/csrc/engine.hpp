@@ -129,6 +129,9 @@ public:
     // y = S x on device buffers (x, y may alias for the spectral kinds)
     void apply(const double* x, double* y, hipStream_t s);
     void apply(const float* x, float* y, hipStream_t s);
+    // same, recording 4 events around the three spectral launches
+    void apply_timed(const double* x, double* y, hipStream_t s, hipEvent_t* ev);
+    void apply_timed(const float* x, float* y, hipStream_t s, hipEvent_t* ev);
 
     Kind kind() const { return kind_; }
     int Ns() const { return Ns_; }
@@ -141,7 +144,7 @@ public:
     const double* table() const { return table_.p; }
 
 private:
-    template <typename T> void apply_t(const T* x, T* y, hipStream_t s);
+    template <typename T> void apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev);
     template <typename T> SpectralPlan<T>& plan();
 
     Kind kind_;

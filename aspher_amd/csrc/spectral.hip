@@ -509,11 +509,15 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
                                const T* ktab, int ktab_pitch, double sym_scale,
                                hipStream_t s, RowIn in_mode, RowOut out_mode,
                                const std::uint8_t* mask, const double* scal,
-                               int gmean_idx, double* part) {
+                               int gmean_idx, double* part, hipEvent_t* ev) {
     if (sym == Symbol::build) throw std::invalid_argument("convolve: build symbol");
+    if (ev) ASPHER_HIP(hipEventRecord(ev[0], s));
     rows_fwd(x, x_pitch, in_mode, mask, scal, gmean_idx, nullptr, 0, rows_in, cols_in, s);
+    if (ev) ASPHER_HIP(hipEventRecord(ev[1], s));
     cols(sym, ktab, ktab_pitch, sym_scale, nullptr, nullptr, rows_in, rows_out, s);
+    if (ev) ASPHER_HIP(hipEventRecord(ev[2], s));
     rows_inv(y, y_pitch, out_mode, mask, part, s);
+    if (ev) ASPHER_HIP(hipEventRecord(ev[3], s));
 }
 
 template <typename T>

@@ -78,7 +78,7 @@ public:
                   RowOut out_mode = RowOut::plain,
                   const std::uint8_t* mask = nullptr,
                   const double* scal = nullptr, int gmean_idx = 0,
-                  double* part = nullptr);
+                  double* part = nullptr, hipEvent_t* stage_events = nullptr);
 
     // Forward-only pass used to build the free-space kernel spectrum from
     // the Love table (T = double): writes Re/n^2 into kout[(kx)(n/2+1)+ky]

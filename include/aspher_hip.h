@@ -64,6 +64,13 @@ int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int pr
 int aspher_solver_time_matvec(aspher_solver* s, const void* x, void* y, int precision,
                               int reps, double* ms_per_matvec);
 
+/* Average device time (ms) of each launch of the spectral matvec pipeline
+ * (ms[0] row forward transform, ms[1] fused column pass, ms[2] row inverse
+ * transform) over `reps` back-to-back matvecs, timed with HIP events on the
+ * library's stream. No reference counterpart (profiling support). */
+int aspher_solver_time_stages(aspher_solver* s, const void* x, void* y, int precision,
+                              int reps, double* ms3);
+
 /* ContactSolver.solve (python/bindings.cpp:186-234, defaults :818-828):
  * precond "none" | "" | "fourier" (13-tap stencil) | "fourier-fft" (full-grid
  * |k|); p_init may be NULL. */
@@ -194,6 +201,14 @@ int aspher_precond_apply(int grid_size, const char* kind, int radius, int precis
                          const double* g, const uint8_t* mask, double* z);
 /* Drop the process-wide operator cache. */
 int aspher_clear_cache(void);
+
+/* Device buffers for callers of the *_device entry points that have no
+ * allocator of their own (tests, benchmarks). No reference counterpart: the
+ * reference is host-only. */
+int aspher_device_malloc(int64_t bytes, void** ptr);
+int aspher_device_free(void* ptr);
+int aspher_memcpy_htod(void* dst, const void* src, int64_t bytes);
+int aspher_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
 
 #ifdef __cplusplus
 }

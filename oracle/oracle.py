@@ -109,12 +109,17 @@ def dense_matrix(Ns: int, L: float = 1.0, E_star: float = 1.0) -> np.ndarray:
 
 # ── transforms (src/fft_engine.hpp: unnormalised r2c / c2r) ─────────────────
 
+# scipy.fft worker threads (1 = the serial restatement; the CPU-baseline leg
+# of bench.py raises it to the host's core count)
+WORKERS = 1
+
+
 def _fwd(G):
-    return sfft.rfft2(G)
+    return sfft.rfft2(G, workers=WORKERS)
 
 
 def _inv(C, n):
-    return sfft.irfft2(C, s=(n, n)) * (n * n)
+    return sfft.irfft2(C, s=(n, n), workers=WORKERS) * (n * n)
 
 
 # ── L3: free-space operator (src/fft_operator.cpp:18-163) ───────────────────
