@@ -1,0 +1,255 @@
+// Register-blocked power-of-two FFT for gfx950.
+//
+// A length n = 2^LOGN transform is factored n = N_0 N_1 ... N_{m-1} with
+// N_s = 16 for the leading stages and one trailing 2/4/8 stage. Each thread
+// owns 16 elements; a stage is a batch of in-register DFTs of size N_s, and
+// the only LDS traffic is one exchange between consecutive stages.
+//
+// Position of an element: pos = sum_j d_j S_j with S_j = prod_{i>j} N_i. The
+// group of stage s (the N_s elements one small DFT combines) is every digit
+// but digit s: pos = high*N_s*S_s + v*S_s + low, g = high*S_s + low; thread
+// th handles groups g = th*G_s + q, G_s = 16/N_s.
+//
+// Forward (decimation in frequency): stage s DFT over digit s, then for
+// s < m-1 multiply by W_{N_s S_s}^{v*low}. Natural-order input, output in
+// digit-reversed order: the value at pos is X[k], k = sum_j d_j P_j with
+// P_j = prod_{i<j} N_i. Inverse (decimation in time): stages m-1 .. 0 with
+// the conjugate twiddle applied before each stage s < m-1; it consumes the
+// digit-reversed order and returns natural order. Forward stage m-1 and
+// inverse stage m-1 share their grouping, so a spectral multiply between
+// them needs no exchange.
+#pragma once
+
+#include "common.hpp"
+
+namespace aspher {
+namespace rb {
+
+// cos(pi j / 16), j = 0..8
+__device__ constexpr double kCos16[9] = {
+    1.0,
+    0.98078528040323044912618223613423903697,
+    0.92387953251128675612818318939678828683,
+    0.83146961230254523707878837761790575673,
+    0.70710678118654752440084436210484903928,
+    0.55557023301960222474283081394853287438,
+    0.38268343236508977172845998403039886676,
+    0.19509032201612826784828486847702224093,
+    0.0};
+
+__device__ __forceinline__ constexpr double cospi16(int j) {
+    j = ((j % 32) + 32) % 32;
+    if (j > 16) j = 32 - j;
+    return (j <= 8) ? kCos16[j] : -kCos16[16 - j];
+}
+__device__ __forceinline__ constexpr double sinpi16(int j) { return cospi16(j - 8); }
+
+template <typename T, bool INV>
+__device__ __forceinline__ cx<T> rot90(cx<T> a) {
+    return INV ? cx<T>{-a.im, a.re} : cx<T>{a.im, -a.re};
+}
+
+// a * W_32^j (forward: exp(-2 pi i j/32); inverse: conjugate). j must fold
+// to a constant after unrolling.
+template <typename T, bool INV>
+__device__ __forceinline__ cx<T> w32(cx<T> a, int j) {
+    j = ((j % 32) + 32) % 32;
+    if (j == 0) return a;
+    if (j == 8) return rot90<T, INV>(a);
+    if (j == 16) return cx<T>{-a.re, -a.im};
+    if (j == 24) return rot90<T, !INV>(a);
+    const T c = static_cast<T>(cospi16(j)), s = static_cast<T>(sinpi16(j));
+    return INV ? cx<T>{a.re * c - a.im * s, a.re * s + a.im * c}
+               : cx<T>{a.re * c + a.im * s, a.im * c - a.re * s};
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft2(cx<T>* a) {
+    const cx<T> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cx<T>* a) {
+    const cx<T> s0 = a[0] + a[2], d0 = a[0] - a[2];
+    const cx<T> s1 = a[1] + a[3], d1 = rot90<T, INV>(a[1] - a[3]);
+    a[0] = s0 + s1;
+    a[2] = s0 - s1;
+    a[1] = d0 + d1;
+    a[3] = d0 - d1;
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cx<T>* a) {
+    cx<T> e[4] = {a[0], a[2], a[4], a[6]};
+    cx<T> o[4] = {a[1], a[3], a[5], a[7]};
+    dft4<T, INV>(e);
+    dft4<T, INV>(o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const cx<T> t = w32<T, INV>(o[k], 4 * k);
+        a[k] = e[k] + t;
+        a[k + 4] = e[k] - t;
+    }
+}
+
+// 16-point DFT as 4 x 4 (decimation in frequency), natural order in and out.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(cx<T>* a) {
+    cx<T> b[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cx<T> t[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+        dft4<T, INV>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) b[4 * k1 + n2] = w32<T, INV>(t[k1], 2 * k1 * n2);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cx<T> t[4] = {b[4 * k1], b[4 * k1 + 1], b[4 * k1 + 2], b[4 * k1 + 3]};
+        dft4<T, INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = t[k2];
+    }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(cx<T>* a) {
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) dft2<T, INV>(a);
+    else if constexpr (R == 4) dft4<T, INV>(a);
+    else if constexpr (R == 8) dft8<T, INV>(a);
+    else dft16<T, INV>(a);
+}
+
+template <int LOGN>
+struct Shape {
+    static_assert(LOGN >= 4 && LOGN <= 13, "register-blocked sizes 16 .. 8192");
+    static constexpr int n = 1 << LOGN;
+    static constexpr int A = LOGN / 4;
+    static constexpr int RL = LOGN % 4;
+    static constexpr int m = A + (RL ? 1 : 0);
+    static constexpr int threads = n / 16;
+    static constexpr int N(int s) { return s < A ? 16 : (1 << RL); }
+    static constexpr int S(int s) {
+        int p = 1;
+        for (int j = s + 1; j < m; ++j) p *= N(j);
+        return p;
+    }
+    static constexpr int P(int s) {
+        int p = 1;
+        for (int j = 0; j < s; ++j) p *= N(j);
+        return p;
+    }
+    static constexpr int G(int s) { return 16 / N(s); }
+    // position of element v of group g in stage s
+    static __device__ __forceinline__ int pos(int s, int g, int v) {
+        const int Ss = S(s), Ns = N(s);
+        return (g / Ss) * Ns * Ss + v * Ss + (g % Ss);
+    }
+    // frequency index of a digit-reversed position
+    static __device__ __forceinline__ int kidx(int p) {
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < m; ++j) k += ((p / S(j)) % N(j)) * P(j);
+        return k;
+    }
+};
+
+// LDS index with a skew of one element every 32 (breaks power-of-two bank
+// strides of the exchanges)
+__device__ __forceinline__ int lds_ix(int p) { return p + (p >> 5); }
+constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
+
+// twiddles W^{v*low*mul}, v = 0..N-1, applied to a group: one FP64 table
+// read, powers as a running FP64 product (at most 15 roundings, ~1e-15
+// relative), rounded once to T
+template <typename T, int NS, bool INV>
+__device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<double>* __restrict__ tw,
+                                              int idx) {
+    if (idx == 0) return;
+    cx<double> w = tw[idx];
+    if (INV) w.im = -w.im;
+    cx<double> wv = w;
+#pragma unroll
+    for (int v = 1; v < NS; ++v) {
+        if (v > 1) wv = wv * w;
+        a[v] = a[v] * cx<T>{static_cast<T>(wv.re), static_cast<T>(wv.im)};
+    }
+}
+
+// Stage s of the forward (INV=false, DFT then twiddle) or inverse (INV=true,
+// twiddle then DFT) transform on the registers of thread th.
+template <typename T, int LOGN, int s, bool INV>
+__device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __restrict__ tw,
+                                      int twmul) {
+    using Sh = Shape<LOGN>;
+    constexpr int Ns = Sh::N(s), Ss = Sh::S(s), Gs = Sh::G(s);
+    constexpr int tstep = Sh::n / (Ns * Ss);     // table stride of W_{Ns Ss}
+#pragma unroll
+    for (int q = 0; q < Gs; ++q) {
+        cx<T>* a = r + q * Ns;
+        const int low = (th * Gs + q) % Ss;
+        if constexpr (INV) {
+            if constexpr (s < Sh::m - 1) twiddle_group<T, Ns, true>(a, tw, low * tstep * twmul);
+            dft<T, Ns, true>(a);
+        } else {
+            dft<T, Ns, false>(a);
+            if constexpr (s < Sh::m - 1) twiddle_group<T, Ns, false>(a, tw, low * tstep * twmul);
+        }
+    }
+}
+
+// Regroup registers from stage sa's grouping to stage sb's through LDS.
+template <typename T, int LOGN, int sa, int sb>
+__device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th) {
+    using Sh = Shape<LOGN>;
+    constexpr int Na = Sh::N(sa), Ga = Sh::G(sa), Nb = Sh::N(sb), Gb = Sh::G(sb);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Ga; ++q)
+#pragma unroll
+        for (int v = 0; v < Na; ++v) lds[lds_ix(Sh::pos(sa, th * Ga + q, v))] = r[q * Na + v];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Gb; ++q)
+#pragma unroll
+        for (int v = 0; v < Nb; ++v) r[q * Nb + v] = lds[lds_ix(Sh::pos(sb, th * Gb + q, v))];
+}
+
+template <typename T, int LOGN, int s>
+__device__ __forceinline__ void forward_from(cx<T>* r, cx<T>* lds, int th,
+                                             const cx<double>* __restrict__ tw, int twmul) {
+    stage<T, LOGN, s, false>(r, th, tw, twmul);
+    if constexpr (s + 1 < Shape<LOGN>::m) {
+        exchange<T, LOGN, s, s + 1>(r, lds, th);
+        forward_from<T, LOGN, s + 1>(r, lds, th, tw, twmul);
+    }
+}
+
+template <typename T, int LOGN, int s>
+__device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
+                                             const cx<double>* __restrict__ tw, int twmul) {
+    stage<T, LOGN, s, true>(r, th, tw, twmul);
+    if constexpr (s > 0) {
+        exchange<T, LOGN, s, s - 1>(r, lds, th);
+        inverse_from<T, LOGN, s - 1>(r, lds, th, tw, twmul);
+    }
+}
+
+// forward: registers in stage-0 grouping (natural order) -> stage m-1
+// grouping (digit-reversed); inverse: the reverse.
+template <typename T, int LOGN>
+__device__ __forceinline__ void forward(cx<T>* r, cx<T>* lds, int th,
+                                        const cx<double>* __restrict__ tw, int twmul) {
+    forward_from<T, LOGN, 0>(r, lds, th, tw, twmul);
+}
+template <typename T, int LOGN>
+__device__ __forceinline__ void inverse(cx<T>* r, cx<T>* lds, int th,
+                                        const cx<double>* __restrict__ tw, int twmul) {
+    inverse_from<T, LOGN, Shape<LOGN>::m - 1>(r, lds, th, tw, twmul);
+}
+
+} // namespace rb
+} // namespace aspher

@@ -12,6 +12,7 @@
 #include "fft_core.hpp"
 #include "reduce.hpp"
 #include "spectral.hpp"
+#include "spectral_rb.hpp"
 
 #include <cmath>
 #include <map>
@@ -359,6 +360,29 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
     logL_row_ = ilog2(Lr);
     xcd_remap_ = tune.xcd_remap;
     const std::size_t cs = sizeof(cx<T>);
+    rb_ = is_pow2(n) && n >= 32 && n <= 8192;
+    if (rb_) {
+        // threads per row/column are L/16 resp. n/16; aim for 512-thread
+        // workgroups while the LDS of one workgroup stays <= 96 KiB
+        const std::size_t rowlds = cs * (std::size_t)(Lr + Lr / 32 + 1);
+        const std::size_t collds = cs * (std::size_t)(n + n / 32 + 1);
+        int B = tune.row_batch > 0 ? tune.row_batch : std::max(1, 512 / (Lr / 16));
+        while (B > 1 && (B * (Lr / 16) > 512 || B * rowlds > 98304)) --B;
+        int W = tune.col_width > 0 ? tune.col_width : std::max(1, 512 / (n / 16));
+        while (W > 1 && (W * (n / 16) > 512 || W * collds > 98304)) --W;
+        if (B * (Lr / 16) > 512 || W * (n / 16) > 512 || B * rowlds > kMaxLds ||
+            W * collds > kMaxLds)
+            throw std::invalid_argument("spectral tiling exceeds the workgroup limits");
+        row_batch_ = B;
+        col_width_ = W;
+        pitchT_ = (std::max(rows_in, 1) + 31) / 32 * 32 + 32;
+        tw64_ = std::make_unique<Twiddles<double>>(n);
+        tw_ = std::make_unique<Twiddles<T>>(n);
+        a_rows_ = static_cast<std::size_t>(std::max(rows_out, 1));
+        ASPHER_HIP(hipMalloc(&A_, cs * a_rows_ * pitch));
+        ASPHER_HIP(hipMalloc(&AT_, cs * (std::size_t)nh * pitchT_));
+        return;
+    }
     if (pow2_) {
         row_batch_ = tune.row_batch > 0 ? tune.row_batch : std::max(1, 2048 / Lr);
         while (Lr * row_batch_ > 8192) row_batch_ /= 2;
@@ -387,11 +411,13 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
 template <typename T>
 SpectralPlan<T>::~SpectralPlan() {
     if (A_) (void)hipFree(A_);
+    if (AT_) (void)hipFree(AT_);
 }
 
 template <typename T>
 std::size_t SpectralPlan<T>::workspace_bytes() const {
-    return sizeof(cx<T>) * a_rows_ * pitch + sizeof(cx<T>) * n;
+    return sizeof(cx<T>) * a_rows_ * pitch + sizeof(cx<T>) * n +
+           (rb_ ? sizeof(cx<T>) * (std::size_t)nh * pitchT_ + sizeof(cx<double>) * n : 0);
 }
 
 template <typename T>
@@ -511,6 +537,59 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
                                const std::uint8_t* mask, const double* scal,
                                int gmean_idx, double* part, hipEvent_t* ev) {
     if (sym == Symbol::build) throw std::invalid_argument("convolve: build symbol");
+    if (rb_) {
+        const int Lr = n / 2;
+        RbRowsFwd<T> f;
+        f.x = x;
+        f.x_pitch = x_pitch;
+        f.ncols = cols_in;
+        f.nrows = rows_in;
+        f.AT = AT_;
+        f.pitchT = pitchT_;
+        f.B = row_batch_;
+        f.tw = reinterpret_cast<const cx<double>*>(tw64_->dev);
+        f.mode = in_mode == RowIn::masked ? 1 : 0;
+        f.mask = mask;
+        f.scal = scal;
+        f.gmean_idx = gmean_idx;
+        f.remap = xcd_remap_ ? 1 : 0;
+        RbCols<T> c;
+        c.AT = AT_;
+        c.pitchT = pitchT_;
+        c.BR = A_;
+        c.pitch = pitch;
+        c.ncols = nh;
+        c.rows_in = rows_in;
+        c.rows_out = rows_out;
+        c.W = col_width_;
+        c.tw = f.tw;
+        c.sym = static_cast<int>(sym);
+        c.ktab = ktab;
+        c.kp = ktab_pitch;
+        c.sc = sym_scale;
+        c.remap = f.remap;
+        RbRowsInv<T> r;
+        r.BR = A_;
+        r.pitch = pitch;
+        r.nrows = rows_out;
+        r.B = row_batch_;
+        r.tw = f.tw;
+        r.y = y;
+        r.y_pitch = y_pitch;
+        r.ncols = cols_out;
+        r.mode = out_mode == RowOut::masked ? 1 : 0;
+        r.mask = mask;
+        r.part = part;
+        r.remap = f.remap;
+        if (ev) ASPHER_HIP(hipEventRecord(ev[0], s));
+        rb_rows_fwd<T>(ilog2(Lr), f, s);
+        if (ev) ASPHER_HIP(hipEventRecord(ev[1], s));
+        rb_cols<T>(logn_, c, s);
+        if (ev) ASPHER_HIP(hipEventRecord(ev[2], s));
+        rb_rows_inv<T>(ilog2(Lr), r, s);
+        if (ev) ASPHER_HIP(hipEventRecord(ev[3], s));
+        return;
+    }
     if (ev) ASPHER_HIP(hipEventRecord(ev[0], s));
     rows_fwd(x, x_pitch, in_mode, mask, scal, gmean_idx, nullptr, 0, rows_in, cols_in, s);
     if (ev) ASPHER_HIP(hipEventRecord(ev[1], s));
@@ -524,6 +603,35 @@ template <typename T>
 void SpectralPlan<T>::build_spectrum(const double* table, int Ns, T* kout, double* part,
                                      hipStream_t s) {
     if (rows_in != n) throw std::invalid_argument("build_spectrum: plan must read all rows");
+    if (rb_) {
+        RbRowsFwd<T> f;
+        f.ncols = n;
+        f.nrows = n;
+        f.AT = AT_;
+        f.pitchT = pitchT_;
+        f.B = row_batch_;
+        f.tw = reinterpret_cast<const cx<double>*>(tw64_->dev);
+        f.mode = 2;
+        f.table = table;
+        f.Ns_embed = Ns;
+        f.remap = xcd_remap_ ? 1 : 0;
+        RbCols<T> c;
+        c.AT = AT_;
+        c.pitchT = pitchT_;
+        c.ncols = nh;
+        c.rows_in = n;
+        c.rows_out = 0;
+        c.W = col_width_;
+        c.tw = f.tw;
+        c.sym = 3;
+        c.kp = nh;
+        c.kout = kout;
+        c.part = part;
+        c.remap = f.remap;
+        rb_rows_fwd<T>(ilog2(n / 2), f, s);
+        rb_cols<T>(logn_, c, s);
+        return;
+    }
     rows_fwd(nullptr, 0, RowIn::embed, nullptr, nullptr, 0, table, Ns, n, n, s);
     cols(Symbol::build, nullptr, nh, 0.0, kout, part, n, 0, s);
 }

@@ -1,0 +1,402 @@
+// Power-of-two spectral convolution kernels built on the register-blocked
+// FFT (fft_rb.hpp). Data flow of y = inv(s . fwd(x)) on an n x n grid:
+//
+//   rows_fwd : x rows (contiguous reads) -> half spectrum written
+//              TRANSPOSED, AT[kx][iy] (strided writes, merged in L2)
+//   cols     : AT columns (contiguous reads) -> forward, symbol, inverse ->
+//              BR[iy][kx] row-major (strided writes)
+//   rows_inv : BR rows (contiguous reads) -> y rows (contiguous writes)
+//
+// Every strided access is a store: stores retire without stalling the wave
+// and the partial lines of neighbouring rows/columns, launched on the same
+// XCD by the XCD-aware block mapping, merge in that XCD's L2 before they
+// reach HBM; every load is a full, coalesced line.
+#include "fft_rb.hpp"
+#include "reduce.hpp"
+#include "spectral_rb.hpp"
+
+#include <map>
+#include <mutex>
+
+namespace aspher {
+
+namespace {
+
+__device__ __forceinline__ int xcd_block(int b, int G) {
+    const int q = G >> 3, r = G & 7;
+    const int x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// ── rows: real -> half spectrum (transposed store) ─────────────────────────
+template <typename T, int LOGL, int IN>
+__global__ void __launch_bounds__(512)
+k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
+              cx<T>* __restrict__ AT, int pitchT, int B, const cx<double>* __restrict__ tw,
+              const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
+              int gmean_idx, const double* __restrict__ table, int Ns_embed, int remap) {
+    using Sh = rb::Shape<LOGL>;
+    constexpr int L = Sh::n, n = 2 * L, TPR = Sh::threads;
+    constexpr int LL = rb::lds_len(L);
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
+    const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = threadIdx.x / TPR, th = threadIdx.x - b * TPR;
+    const int row = blk * B + b;
+    cx<T>* lds = lds_all + b * LL;
+    T gmean = T(0);
+    if constexpr (IN == 1) gmean = static_cast<T>(scal[gmean_idx]);
+
+    auto load = [&](int col) -> T {
+        if (row >= nrows || col >= ncols) return T(0);
+        if constexpr (IN == 2) {
+            const int Ns = Ns_embed;
+            if (row == Ns || col == Ns) return T(0);
+            const int ady = row < Ns ? row : n - row;
+            const int adx = col < Ns ? col : n - col;
+            return static_cast<T>(table[(size_t)ady * Ns + adx]);
+        } else {
+            const size_t i = (size_t)row * x_pitch + col;
+            if constexpr (IN == 1) return mask[i] ? T(x[i] - gmean) : T(0);
+            else return x[i];
+        }
+    };
+
+    cx<T> r[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int p = Sh::pos(0, th, v);   // packed index j: reals 2j, 2j+1
+        r[v] = {load(2 * p), load(2 * p + 1)};
+    }
+    rb::forward<T, LOGL>(r, lds, th, tw, 2);
+    // digit-reversed registers -> natural order in LDS, then split the
+    // packed spectrum: X[k] = (Z_k + conj Z_{L-k})/2 - i W^k (Z_k - conj Z_{L-k})/2
+    __syncthreads();
+    constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
+#pragma unroll
+    for (int q = 0; q < Gl; ++q)
+#pragma unroll
+        for (int v = 0; v < Nl; ++v)
+            lds[rb::lds_ix(Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v)))] = r[q * Nl + v];
+    __syncthreads();
+    // store: rows of the workgroup fastest, so B consecutive rows of one kx
+    // form one contiguous run of the transposed layout
+    const int nb = (nrows - blk * B < B) ? (nrows - blk * B) : B;
+    for (int e = threadIdx.x; e < B * (L + 1); e += blockDim.x) {
+        const int bb = e % B, k = e / B;
+        if (bb >= nb) continue;
+        const cx<T>* lb = lds_all + bb * LL;
+        const cx<T> Zk = lb[rb::lds_ix(k == L ? 0 : k)];
+        const cx<T> Zc = conjg(lb[rb::lds_ix(k == 0 ? 0 : L - k)]);
+        const cx<T> E = scale(Zk + Zc, T(0.5));
+        const cx<T> D = Zk - Zc;
+        const cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};
+        const cx<double> w = tw[k];
+        const cx<T> X = E + cx<T>{static_cast<T>(w.re), static_cast<T>(w.im)} * O;
+        AT[(size_t)k * pitchT + blk * B + bb] = X;
+    }
+}
+
+// ── columns: forward . symbol . inverse (transposed in, row-major out) ─────
+template <typename T, int SYM>
+__device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restrict__ ktab,
+                                       int kp, double sc) {
+    if constexpr (SYM == 0) {
+        const int kyp = (ky <= (n >> 1)) ? ky : n - ky;
+        return ktab[(size_t)kx * kp + kyp];
+    } else if constexpr (SYM == 1) {
+        const int my = (ky < ((n + 1) >> 1)) ? ky : ky - n;
+        const double m2 = (double)kx * kx + (double)my * my;
+        return m2 > 0.0 ? static_cast<T>(sc / sqrt(m2)) : T(0);
+    } else {
+        const int kof = (ky < (n >> 1)) ? ky : ky - n;
+        const float h = (float)sqrt((double)kx * kx + (double)kof * kof);
+        const float w = (kx == 0 && ky == 0) ? 0.0f : (float)sc * h;
+        return static_cast<T>(w);
+    }
+}
+
+template <typename T, int LOGN, int SYM>
+__global__ void __launch_bounds__(512)
+k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int pitch,
+          int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
+          const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
+          double* __restrict__ part, int remap) {
+    using Sh = rb::Shape<LOGN>;
+    constexpr int n = Sh::n, TPC = Sh::threads;
+    constexpr int LL = rb::lds_len(n);
+    constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[16];
+    const int tile = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int c = threadIdx.x / TPC, th = threadIdx.x - c * TPC;
+    const int kx = tile * W + c;
+    const bool live = kx < ncols;
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem) + c * LL;
+
+    cx<T> r[16];
+    const cx<T>* col = AT + (size_t)(live ? kx : 0) * pitchT;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int p = Sh::pos(0, th, v);
+        r[v] = (live && p < rows_in) ? col[p] : cx<T>{T(0), T(0)};
+    }
+    rb::forward<T, LOGN>(r, lds, th, tw, 1);
+
+    if constexpr (SYM == 3) {
+        // kernel-spectrum build (src/fft_operator.cpp:46-68): realness check
+        // over every mode, Re/n^2 stored for ky in [0, n/2] (even in ky)
+        const double norm = 1.0 / ((double)n * (double)n);
+        double mi = 0.0, mr = 0.0;
+#pragma unroll
+        for (int q = 0; q < Gl; ++q)
+#pragma unroll
+            for (int v = 0; v < Nl; ++v) {
+                const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+                const cx<T> z = r[q * Nl + v];
+                if (live) {
+                    mi = fmax(mi, fabs((double)z.im));
+                    mr = fmax(mr, fabs((double)z.re));
+                    if (ky <= (n >> 1)) kout[(size_t)kx * kp + ky] = static_cast<T>(norm * (double)z.re);
+                }
+            }
+        mi = block_reduce<RMAX>(mi, red);
+        if (threadIdx.x == 0) part[2 * blockIdx.x] = mi;
+        mr = block_reduce<RMAX>(mr, red);
+        if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = mr;
+        return;
+    } else {
+#pragma unroll
+        for (int q = 0; q < Gl; ++q)
+#pragma unroll
+            for (int v = 0; v < Nl; ++v) {
+                const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+                const T s = live ? symbol_rb<T, SYM>(kx, ky, n, ktab, kp, sc) : T(0);
+                r[q * Nl + v] = scale(r[q * Nl + v], s);
+            }
+        rb::inverse<T, LOGN>(r, lds, th, tw, 1);
+        if (live) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int p = Sh::pos(0, th, v);
+                if (p < rows_out) BR[(size_t)p * pitch + kx] = r[v];
+            }
+        }
+    }
+}
+
+// ── rows: half spectrum (row-major) -> real ────────────────────────────────
+template <typename T, int LOGL, int OUT>
+__global__ void __launch_bounds__(512)
+k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
+              const cx<double>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
+              const std::uint8_t* __restrict__ mask, double* __restrict__ part, int remap) {
+    using Sh = rb::Shape<LOGL>;
+    constexpr int L = Sh::n, TPR = Sh::threads;
+    constexpr int LL = rb::lds_len(L);
+    constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[16];
+    cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
+    const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = threadIdx.x / TPR, th = threadIdx.x - b * TPR;
+    const int row = blk * B + b;
+    cx<T>* lds = lds_all + b * LL;
+    // coalesced load of the L+1 half-spectrum values of each row
+    for (int e = threadIdx.x; e < B * (L + 1); e += blockDim.x) {
+        const int bb = e / (L + 1), k = e - bb * (L + 1);
+        const int rr = blk * B + bb;
+        lds_all[bb * LL + rb::lds_ix(k)] =
+            rr < nrows ? BR[(size_t)rr * pitch + k] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    // Z_k = (X_k + conj X_{L-k}) + i (X_k - conj X_{L-k}) exp(+2 pi i k/n),
+    // gathered straight into the digit-reversed grouping of the inverse
+    cx<T> r[16];
+#pragma unroll
+    for (int q = 0; q < Gl; ++q)
+#pragma unroll
+        for (int v = 0; v < Nl; ++v) {
+            const int k = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+            cx<T> Xk = lds[rb::lds_ix(k)], Xc = lds[rb::lds_ix(L - k)];
+            if (k == 0) {
+                Xk.im = T(0);
+                Xc.im = T(0);
+            }
+            Xc = conjg(Xc);
+            const cx<T> S = Xk + Xc;
+            const cx<double> w = tw[k];
+            const cx<T> D = (Xk - Xc) * cx<T>{static_cast<T>(w.re), static_cast<T>(-w.im)};
+            r[q * Nl + v] = {S.re - D.im, S.im + D.re};
+        }
+    rb::inverse<T, LOGL>(r, lds, th, tw, 2);
+    double acc = 0.0;
+    if (row < nrows) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int p = Sh::pos(0, th, v);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int col = 2 * p + h;
+                if (col >= ncols) continue;
+                T val = h ? r[v].im : r[v].re;
+                const size_t i = (size_t)row * y_pitch + col;
+                if constexpr (OUT == 1) {
+                    if (mask[i]) acc += (double)val;
+                    else val = T(0);
+                }
+                y[i] = val;
+            }
+        }
+    }
+    if constexpr (OUT == 1) {
+        acc = block_reduce<RSUM>(acc, red);
+        if (threadIdx.x == 0) part[(std::size_t)blockIdx.x * kPartVals] = acc;
+    }
+}
+
+void allow_lds_rb(const void* fn, std::size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, std::size_t> done;
+    if (bytes <= 65536) return;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(fn);
+    if (it != done.end() && it->second >= bytes) return;
+    ASPHER_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(bytes)));
+    done[fn] = bytes;
+}
+
+template <typename T, int LOGL>
+void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
+    constexpr int L = 1 << LOGL;
+    const int block = a.B * (L / 16);
+    const int grid = (a.nrows + a.B - 1) / a.B;
+    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
+    const void* fn;
+#define RF_ARGS a.x, a.x_pitch, a.ncols, a.nrows, a.AT, a.pitchT, a.B, a.tw, a.mask, a.scal, \
+        a.gmean_idx, a.table, a.Ns_embed, a.remap
+    if (a.mode == 0) {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 0>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_fwd<T, LOGL, 0><<<grid, block, lds, s>>>(RF_ARGS);
+    } else if (a.mode == 1) {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 1>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_fwd<T, LOGL, 1><<<grid, block, lds, s>>>(RF_ARGS);
+    } else {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 2>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_fwd<T, LOGL, 2><<<grid, block, lds, s>>>(RF_ARGS);
+    }
+#undef RF_ARGS
+    ASPHER_LAUNCH_CHECK();
+}
+
+template <typename T, int LOGN>
+void launch_cols(const RbCols<T>& a, hipStream_t s) {
+    constexpr int n = 1 << LOGN;
+    const int block = a.W * (n / 16);
+    const int grid = (a.ncols + a.W - 1) / a.W;
+    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.W * rb::lds_len(n);
+#define RC_ARGS a.AT, a.pitchT, a.BR, a.pitch, a.ncols, a.rows_in, a.rows_out, a.W, a.tw, a.ktab, \
+        a.kp, a.sc, a.kout, a.part, a.remap
+#define RC_CASE(S)                                                     \
+    case S: {                                                          \
+        const void* fn = (const void*)k_rb_cols<T, LOGN, S>;           \
+        allow_lds_rb(fn, lds);                                         \
+        k_rb_cols<T, LOGN, S><<<grid, block, lds, s>>>(RC_ARGS);       \
+        break;                                                         \
+    }
+    switch (a.sym) {
+        RC_CASE(0)
+        RC_CASE(1)
+        RC_CASE(2)
+        RC_CASE(3)
+        default: throw std::invalid_argument("unknown symbol");
+    }
+#undef RC_CASE
+#undef RC_ARGS
+    ASPHER_LAUNCH_CHECK();
+}
+
+template <typename T, int LOGL>
+void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
+    constexpr int L = 1 << LOGL;
+    const int block = a.B * (L / 16);
+    const int grid = (a.nrows + a.B - 1) / a.B;
+    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
+#define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap
+    if (a.mode == 0) {
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_inv<T, LOGL, 0><<<grid, block, lds, s>>>(RI_ARGS);
+    } else {
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 1>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_inv<T, LOGL, 1><<<grid, block, lds, s>>>(RI_ARGS);
+    }
+#undef RI_ARGS
+    ASPHER_LAUNCH_CHECK();
+}
+
+} // namespace
+
+template <typename T>
+void rb_rows_fwd(int logL, const RbRowsFwd<T>& a, hipStream_t s) {
+    switch (logL) {
+        case 4: launch_rows_fwd<T, 4>(a, s); break;
+        case 5: launch_rows_fwd<T, 5>(a, s); break;
+        case 6: launch_rows_fwd<T, 6>(a, s); break;
+        case 7: launch_rows_fwd<T, 7>(a, s); break;
+        case 8: launch_rows_fwd<T, 8>(a, s); break;
+        case 9: launch_rows_fwd<T, 9>(a, s); break;
+        case 10: launch_rows_fwd<T, 10>(a, s); break;
+        case 11: launch_rows_fwd<T, 11>(a, s); break;
+        case 12: launch_rows_fwd<T, 12>(a, s); break;
+        case 13: launch_rows_fwd<T, 13>(a, s); break;
+        default: throw std::invalid_argument("row transform length out of range");
+    }
+}
+
+template <typename T>
+void rb_cols(int logN, const RbCols<T>& a, hipStream_t s) {
+    switch (logN) {
+        case 5: launch_cols<T, 5>(a, s); break;
+        case 6: launch_cols<T, 6>(a, s); break;
+        case 7: launch_cols<T, 7>(a, s); break;
+        case 8: launch_cols<T, 8>(a, s); break;
+        case 9: launch_cols<T, 9>(a, s); break;
+        case 10: launch_cols<T, 10>(a, s); break;
+        case 11: launch_cols<T, 11>(a, s); break;
+        case 12: launch_cols<T, 12>(a, s); break;
+        case 13: launch_cols<T, 13>(a, s); break;
+        default: throw std::invalid_argument("column transform length out of range");
+    }
+}
+
+template <typename T>
+void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s) {
+    switch (logL) {
+        case 4: launch_rows_inv<T, 4>(a, s); break;
+        case 5: launch_rows_inv<T, 5>(a, s); break;
+        case 6: launch_rows_inv<T, 6>(a, s); break;
+        case 7: launch_rows_inv<T, 7>(a, s); break;
+        case 8: launch_rows_inv<T, 8>(a, s); break;
+        case 9: launch_rows_inv<T, 9>(a, s); break;
+        case 10: launch_rows_inv<T, 10>(a, s); break;
+        case 11: launch_rows_inv<T, 11>(a, s); break;
+        case 12: launch_rows_inv<T, 12>(a, s); break;
+        case 13: launch_rows_inv<T, 13>(a, s); break;
+        default: throw std::invalid_argument("row transform length out of range");
+    }
+}
+
+template void rb_rows_fwd<float>(int, const RbRowsFwd<float>&, hipStream_t);
+template void rb_rows_fwd<double>(int, const RbRowsFwd<double>&, hipStream_t);
+template void rb_cols<float>(int, const RbCols<float>&, hipStream_t);
+template void rb_cols<double>(int, const RbCols<double>&, hipStream_t);
+template void rb_rows_inv<float>(int, const RbRowsInv<float>&, hipStream_t);
+template void rb_rows_inv<double>(int, const RbRowsInv<double>&, hipStream_t);
+
+} // namespace aspher

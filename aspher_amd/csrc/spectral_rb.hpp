@@ -1,0 +1,63 @@
+// Launch interface of the register-blocked power-of-two spectral kernels.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstdint>
+
+namespace aspher {
+
+template <typename T>
+struct RbRowsFwd {
+    const T* x = nullptr;
+    int x_pitch = 0, ncols = 0, nrows = 0;
+    cx<T>* AT = nullptr;        // transposed half spectrum, AT[kx*pitchT + iy]
+    int pitchT = 0;
+    int B = 1;                  // rows per workgroup
+    const cx<double>* tw = nullptr;   // exp(-2 pi i t/n), n = 2L
+    int mode = 0;               // 0 plain, 1 masked (mask ? x - gmean : 0), 2 Love embedding
+    const std::uint8_t* mask = nullptr;
+    const double* scal = nullptr;
+    int gmean_idx = 0;
+    const double* table = nullptr;
+    int Ns_embed = 0;
+    int remap = 1;
+};
+
+template <typename T>
+struct RbCols {
+    const cx<T>* AT = nullptr;
+    int pitchT = 0;
+    cx<T>* BR = nullptr;        // row-major output BR[iy*pitch + kx]
+    int pitch = 0;
+    int ncols = 0, rows_in = 0, rows_out = 0;
+    int W = 1;                  // columns per workgroup
+    const cx<double>* tw = nullptr;   // exp(-2 pi i t/n)
+    int sym = 0;                // 0 table, 1 periodic, 2 precond, 3 build
+    const T* ktab = nullptr;
+    int kp = 0;
+    double sc = 0.0;
+    T* kout = nullptr;
+    double* part = nullptr;
+    int remap = 1;
+};
+
+template <typename T>
+struct RbRowsInv {
+    const cx<T>* BR = nullptr;
+    int pitch = 0, nrows = 0;
+    int B = 1;
+    const cx<double>* tw = nullptr;   // exp(-2 pi i t/n), n = 2L
+    T* y = nullptr;
+    int y_pitch = 0, ncols = 0;
+    int mode = 0;               // 0 plain, 1 masked with per-block sum
+    const std::uint8_t* mask = nullptr;
+    double* part = nullptr;
+    int remap = 1;
+};
+
+template <typename T> void rb_rows_fwd(int logL, const RbRowsFwd<T>& a, hipStream_t s);
+template <typename T> void rb_cols(int logN, const RbCols<T>& a, hipStream_t s);
+template <typename T> void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s);
+
+} // namespace aspher
