@@ -283,7 +283,7 @@ void stencil_weights(int Ns, int R, std::vector<int>& dx, std::vector<int>& dy,
     cache[key] = std::make_tuple(dx, dy, w);
 }
 
-Precond::Precond(Kind kind, int Ns, int radius) : kind_(kind), Ns_(Ns) {
+Precond::Precond(Kind kind, int Ns, int radius) : kind_(kind), Ns_(Ns), R_(radius) {
     if (kind_ == Kind::stencil) {
         if (radius < 1) throw std::invalid_argument("stencil radius must be >= 1");
         if (2 * radius + 1 > Ns)
@@ -311,7 +311,7 @@ void Precond::apply(const T* g, const std::uint8_t* mask, T* z, double* scal, do
     static const int dz[1] = {S_ZSUM};
     if (kind_ == Kind::stencil) {
         const int nblk = vec_blocks((std::int64_t)Ns_ * Ns_);
-        VecOps<T>::stencil(Ns_, g, mask, scal, tapd_.p, tapw_.p, (int)w_.size(), z, part,
+        VecOps<T>::stencil(Ns_, g, mask, scal, tapd_.p, tapw_.p, (int)w_.size(), R_, z, part,
                            nblk, s);
         finalize<T>(part, nblk, 1, kSum1, dz, scal, STEP_ZMEAN, s);
     } else if (kind_ == Kind::fourier) {

@@ -180,6 +180,7 @@ public:
 private:
     Kind kind_;
     int Ns_;
+    int R_ = 0;
     std::vector<int> dx_, dy_;
     std::vector<double> w_;
     DevBuf<int> tapd_;

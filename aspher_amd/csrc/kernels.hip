@@ -83,34 +83,54 @@ k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
 }
 
 // ── stencil preconditioner apply_full (src/stencil_precond.cpp:63-109) ────
-// 2-D tiles: a 64 x 4 block of outputs per workgroup; taps read through L1.
+// Tiles of 64 x 16 outputs (grid-stride over tiles): the centred, masked
+// residual of the tile plus an R-cell periodic halo is staged once in LDS
+// (FP64), then every contact point sums its taps from LDS in the
+// reference's tap order.
+constexpr int kStX = 64, kStY = 16;
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask,
           const double* __restrict__ scal, const int* __restrict__ tapd,
-          const double* __restrict__ tapw, int ntaps, T* __restrict__ z,
+          const double* __restrict__ tapw, int ntaps, int R, T* __restrict__ z,
           double* __restrict__ part) {
     __shared__ double red[8];
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* tile = reinterpret_cast<double*>(smem);
+    const int HX = kStX + 2 * R, HY = kStY + 2 * R;
     const double gmean = scal[S_GMEAN];
+    const int ntx = (Ns + kStX - 1) / kStX, nty = (Ns + kStY - 1) / kStY;
+    const int tx = threadIdx.x & 63, ty0 = threadIdx.x >> 6;
     double zsum = 0.0;
-    const std::int64_t N = (std::int64_t)Ns * Ns;
-    GRID_LOOP(i, N) {
-        const int iy = (int)(i / Ns), ix = (int)(i - (std::int64_t)iy * Ns);
-        if (!mask[i]) {
-            z[i] = T(0);
-            continue;
+    for (int t = blockIdx.x; t < ntx * nty; t += gridDim.x) {
+        const int x0 = (t % ntx) * kStX, y0 = (t / ntx) * kStY;
+        __syncthreads();
+        for (int e = threadIdx.x; e < HX * HY; e += blockDim.x) {
+            const int hy = e / HX, hx = e - hy * HX;
+            int jx = (x0 - R + hx) % Ns, jy = (y0 - R + hy) % Ns;
+            if (jx < 0) jx += Ns;
+            if (jy < 0) jy += Ns;
+            const std::size_t j = (std::size_t)jy * Ns + jx;
+            tile[e] = mask[j] ? (double)g[j] - gmean : 0.0;
         }
-        double acc = 0.0;
-        for (int t = 0; t < ntaps; ++t) {
-            int jx = ix + tapd[2 * t], jy = iy + tapd[2 * t + 1];
-            if (jx < 0) jx += Ns; else if (jx >= Ns) jx -= Ns;
-            if (jy < 0) jy += Ns; else if (jy >= Ns) jy -= Ns;
-            const std::int64_t j = (std::int64_t)jy * Ns + jx;
-            const double v = mask[j] ? (double)g[j] - gmean : 0.0;
-            acc += tapw[t] * v;
+        __syncthreads();
+        const int ix = x0 + tx;
+#pragma unroll
+        for (int k = 0; k < kStY / 4; ++k) {
+            const int ty = ty0 + 4 * k, iy = y0 + ty;
+            if (ix >= Ns || iy >= Ns) continue;
+            const std::size_t i = (std::size_t)iy * Ns + ix;
+            if (!mask[i]) {
+                z[i] = T(0);
+                continue;
+            }
+            double acc = 0.0;
+            const double* c = tile + (ty + R) * HX + (tx + R);
+            for (int q = 0; q < ntaps; ++q) acc += tapw[q] * c[tapd[2 * q + 1] * HX + tapd[2 * q]];
+            z[i] = static_cast<T>(acc);
+            zsum += acc;
         }
-        z[i] = static_cast<T>(acc);
-        zsum += acc;
     }
     put<RSUM>(zsum, red, part, 1, 0);
 }
@@ -626,9 +646,11 @@ void VecOps<T>::eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
 }
 template <typename T>
 void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const double* scal,
-                        const int* tapd, const double* tapw, int ntaps, T* z,
+                        const int* tapd, const double* tapw, int ntaps, int R, T* z,
                         double* part, int nblk, hipStream_t s) {
-    L1(k_stencil<T>, Ns, g, mask, scal, tapd, tapw, ntaps, z, part);
+    const std::size_t lds = sizeof(double) * (kStX + 2 * R) * (kStY + 2 * R);
+    k_stencil<T><<<nblk, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, ntaps, R, z, part);
+    ASPHER_LAUNCH_CHECK();
 }
 template <typename T>
 void VecOps<T>::cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,

@@ -41,7 +41,7 @@ template <typename T> struct VecOps {
     // 13-tap (any radius) masked stencil (uses S_GMEAN): z and partial zsum
     static void stencil(int Ns, const T* g, const std::uint8_t* mask,
                         const double* scal, const int* tapd, const double* tapw,
-                        int ntaps, T* z, double* part, int nblk, hipStream_t s);
+                        int ntaps, int R, T* z, double* part, int nblk, hipStream_t s);
     // {G, G_pr, gg} over the mask; z is (z - zmean) when has_pc, else g
     static void cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
                       const std::uint8_t* mask, const double* scal, bool has_pc,

@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--float-floor", type=float, default=None)
     ap.add_argument("--matvec-only", action="store_true",
                     help="only the matvec timing (for profiler runs)")
+    ap.add_argument("--solve-once", choices=("mixed", "fp64"),
+                    help="one cached-operator solve of the given precision (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -131,6 +133,16 @@ def main():
 
     Ns = args.ns
     assert hc.device_count() >= 1, "no GPU visible"
+
+    if args.solve_once:
+        gap = rough_gap(Ns, H=0.8, k_low=4, k_high=64, rms=0.002, seed=0)
+        g = DeviceArray.from_numpy(gap, np.float64)
+        prec = "float_then_double" if args.solve_once == "mixed" else "double"
+        run_solve(hc, Ns, g.ptr, prec, True, args.float_floor)      # builds the operators
+        t0 = time.perf_counter()
+        r = run_solve(hc, Ns, g.ptr, prec, True, args.float_floor)
+        log("[bench] %s solve %.3f s %s" % (args.solve_once, time.perf_counter() - t0, r))
+        return
 
     # ── matvec roofline (FP32 and FP64) ────────────────────────────────────
     log("[bench] matvec timing Ns=%d" % Ns)
