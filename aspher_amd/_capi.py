@@ -26,6 +26,8 @@ EXPORTS = (
     "aspher_love_points", "aspher_love_table", "aspher_stencil_taps",
     "aspher_precond_apply", "aspher_clear_cache", "aspher_device_malloc",
     "aspher_device_free", "aspher_memcpy_htod", "aspher_memcpy_dtoh",
+    "aspher_solver_create_shared", "aspher_adjoint_params_default",
+    "aspher_solver_adjoint", "aspher_solver_adjoint_device",
 )
 
 
@@ -82,6 +84,18 @@ class StageInfo(C.Structure):
                 ("precond_dropped", C.c_int)]
 
 
+class AdjointParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precond", C.c_char * 16)]
+
+
+class AdjointInfo(C.Structure):
+    _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("n_contact", C.c_int64),
+                ("matvec_count", C.c_int64), ("precond_count", C.c_int64),
+                ("rel_residual", C.c_double), ("multiplier", C.c_double),
+                ("grad_p_nominal", C.c_double), ("seconds", C.c_double),
+                ("status", C.c_char * 32), ("precond", C.c_char * 16)]
+
+
 _lib = None
 
 
@@ -122,6 +136,12 @@ def _declare(L):
         "aspher_device_free": (i, [vp]),
         "aspher_memcpy_htod": (i, [vp, vp, C.c_int64]),
         "aspher_memcpy_dtoh": (i, [vp, vp, C.c_int64]),
+        "aspher_solver_create_shared": (i, [i, d, d, s, C.POINTER(vp)]),
+        "aspher_adjoint_params_default": (None, [C.POINTER(AdjointParams)]),
+        "aspher_solver_adjoint": (i, [vp, vp, dp, C.POINTER(AdjointParams), dp,
+                                      C.POINTER(AdjointInfo)]),
+        "aspher_solver_adjoint_device": (i, [vp, vp, vp, C.POINTER(AdjointParams), vp,
+                                             C.POINTER(AdjointInfo)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -144,7 +144,11 @@ class ContactSolver:
 
     def __init__(self, grid_size, domain_size=1.0, E_star=1.0, eta=2.0, aca_tol=1e-6,
                  leaf_size=64, use_hmatrix=True, use_acagp=False, central_fraction=0.3,
-                 inline_svd_tol=0.0, backend="", q=4, near_radius=1, h2_leaf_side=8):
+                 inline_svd_tol=0.0, backend="", q=4, near_radius=1, h2_leaf_side=8,
+                 shared_operator=False):
+        """``shared_operator`` (not in the reference) takes the operator from
+        the process cache that ``solve_nested(reuse_operators=True)`` fills,
+        so an adjoint runs on the forward solve's own kernel spectrum."""
         grid_size = int(grid_size)
         if grid_size <= 0:
             raise ValueError("grid_size must be positive")
@@ -155,8 +159,10 @@ class ContactSolver:
         if not backend:
             backend = "hmatrix" if use_hmatrix else "dense"
         self._h = C.c_void_p()
-        check(lib().aspher_solver_create(grid_size, float(domain_size), float(E_star),
-                                         backend.encode(), C.byref(self._h)))
+        create = (lib().aspher_solver_create_shared if shared_operator
+                  else lib().aspher_solver_create)
+        check(create(grid_size, float(domain_size), float(E_star), backend.encode(),
+                     C.byref(self._h)))
         self.grid_size = grid_size
         self.backend = backend
         self.domain_size = float(domain_size)
@@ -224,6 +230,30 @@ class ContactSolver:
                                                C.c_void_p(p_init_ptr or None), C.byref(out)))
         return ContactResult(out.value)
 
+    def adjoint(self, result: ContactResult, grad_pressure, tol=1e-10, max_iter=5000,
+                precond="auto"):
+        """Backward pass of a converged solve on this grid (no reference
+        counterpart): for dL/dp = grad_pressure returns (dL/dgap as (Ns, Ns),
+        dL/dp_nominal, info dict). Solves [S_CC 1; 1' 0][mu; nu] = [w_C; 0]
+        on the support C = {p > 0} by preconditioned CG with this solver's
+        operator; dL/dgap = -mu on C, 0 elsewhere; dL/dp_nominal = nu Ns^2."""
+        w = _flat(grad_pressure, self.N)
+        g = np.empty(self.N, dtype=np.float64)
+        prm, info = _adjoint_params(tol, max_iter, precond), _capi.AdjointInfo()
+        check(lib().aspher_solver_adjoint(self._h, result._h, dptr(w), C.byref(prm),
+                                          dptr(g), C.byref(info)))
+        return (g.reshape(self.grid_size, self.grid_size), info.grad_p_nominal,
+                _adjoint_info(info))
+
+    def adjoint_device(self, result: ContactResult, grad_pressure_ptr: int,
+                       grad_gap_ptr: int, tol=1e-10, max_iter=5000, precond="auto"):
+        """adjoint() on device FP64 buffers; returns (dL/dp_nominal, info)."""
+        prm, info = _adjoint_params(tol, max_iter, precond), _capi.AdjointInfo()
+        check(lib().aspher_solver_adjoint_device(self._h, result._h,
+                                                 C.c_void_p(grad_pressure_ptr), C.byref(prm),
+                                                 C.c_void_p(grad_gap_ptr), C.byref(info)))
+        return info.grad_p_nominal, _adjoint_info(info)
+
     def hmatrix_info(self) -> dict:
         """Operator statistics (python/bindings.cpp:307-328 for fft/periodic)."""
         info = _capi.OperatorInfo()
@@ -252,6 +282,22 @@ class ContactSolver:
     def recompress(self, svd_tol) -> None:
         """H-matrix recompression: nothing to do for the spectral backends."""
         return None
+
+
+def _adjoint_params(tol, max_iter, precond) -> _capi.AdjointParams:
+    p = _capi.AdjointParams()
+    lib().aspher_adjoint_params_default(C.byref(p))
+    p.tol, p.max_iter = float(tol), int(max_iter)
+    p.precond = str(precond).encode()
+    return p
+
+
+def _adjoint_info(i) -> dict:
+    return dict(converged=bool(i.converged), iterations=i.iterations, n_contact=i.n_contact,
+                matvec_count=i.matvec_count, precond_count=i.precond_count,
+                rel_residual=i.rel_residual, multiplier=i.multiplier,
+                grad_p_nominal=i.grad_p_nominal, seconds=i.seconds,
+                status=i.status.decode(), precond=i.precond.decode())
 
 
 _PRECOND_ENGINES = ("auto", "stencil", "fft")

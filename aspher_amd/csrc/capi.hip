@@ -5,6 +5,7 @@
 #include "kernels.hpp"
 #include "reduce.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -115,6 +116,88 @@ void solve_single(aspher_solver* s, const double* gap_dev, double p_nominal, dou
     *out = res.release();
 }
 
+void create_solver(int grid_size, double domain_size, double E_star, const char* backend,
+                   bool shared, aspher_solver** out) {
+    if (!out) throw std::invalid_argument("null output handle");
+    // validation order of python/bindings.cpp:122-127
+    if (grid_size <= 0) throw std::invalid_argument("grid_size must be positive");
+    if (!(domain_size > 0.0)) throw std::invalid_argument("domain_size must be positive");
+    if (!(E_star > 0.0)) throw std::invalid_argument("E_star must be positive");
+    const std::string b = backend ? backend : "";
+    Operator::Kind kind;
+    if (b == "fft") kind = Operator::Kind::free_space;
+    else if (b == "periodic") kind = Operator::Kind::periodic;
+    else if (b == "dense") kind = Operator::Kind::dense;
+    else if (b == "h2" || b == "hmatrix" || b.empty())
+        throw std::invalid_argument("backend '" + (b.empty() ? std::string("hmatrix") : b) +
+                                    "' is not part of this build (expected fft, periodic "
+                                    "or dense)");
+    else
+        throw std::invalid_argument("unknown backend: " + b +
+                                    " (expected dense, hmatrix, h2, fft, or periodic)");
+    auto s = std::make_unique<aspher_solver>();
+    s->ctx = std::make_unique<Context>();
+    if (shared) {
+        s->op = cached_operator(kind, grid_size, domain_size, E_star, s->ctx->stream);
+    } else {
+        s->op = std::make_shared<Operator>(kind, grid_size, domain_size, E_star);
+        s->op->build(s->ctx->stream);
+    }
+    s->backend = b;
+    s->Ns = grid_size;
+    *out = s.release();
+}
+
+
+AdjointOptions to_adjoint(const aspher_adjoint_params* p) {
+    AdjointOptions o;
+    if (p) {
+        o.tol = p->tol;
+        o.max_iter = p->max_iter;
+    }
+    return o;
+}
+
+// "auto": the full-grid Fourier preconditioner on the periodic backend (the
+// nested driver's periodic default), otherwise the 13-tap stencil unless the
+// support covers more than 40% of the grid (src/nested_solve.cpp:27-40)
+Precond::Kind adjoint_precond(const aspher_solver* s, const char* name, long long n_contact) {
+    const std::string nm = name ? name : "auto";
+    if (nm != "auto") return precond_kind(nm);
+    if (s->backend == "periodic") return Precond::Kind::fourier;
+    const double occ = (double)n_contact / ((double)s->Ns * s->Ns);
+    return occ < 0.4 ? Precond::Kind::stencil : Precond::Kind::fourier;
+}
+
+void adjoint_device(aspher_solver* s, const aspher_result* res, const double* w_dev,
+                    const aspher_adjoint_params* prm, double* grad_dev,
+                    aspher_adjoint_info* info) {
+    if (!s || !res || !w_dev || !grad_dev) throw std::invalid_argument("null pointer");
+    if (res->r.Ns != s->Ns)
+        throw std::invalid_argument("result grid size does not match the solver");
+    if (!res->r.pressure.p) throw std::invalid_argument("result holds no pressure field");
+    const long long nc = (long long)std::llround(res->r.contact_fraction * s->Ns * (double)s->Ns);
+    Precond pc(adjoint_precond(s, prm ? prm->precond : nullptr, nc), s->Ns, 2);
+    AdjointInfo ai;
+    adjoint_impl(*s->ctx, *s->op, &pc, res->r.pressure.p, w_dev, grad_dev, to_adjoint(prm), ai);
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->converged = ai.converged ? 1 : 0;
+        info->iterations = ai.iterations;
+        info->n_contact = ai.n_contact;
+        info->matvec_count = ai.matvec_count;
+        info->precond_count = ai.precond_count;
+        info->rel_residual = ai.rel_residual;
+        info->multiplier = ai.multiplier;
+        info->grad_p_nominal = ai.grad_p_nominal;
+        info->seconds = ai.seconds;
+        copy_str(info->status, sizeof(info->status), ai.status);
+        copy_str(info->precond, sizeof(info->precond),
+                 pc.kind() == Precond::Kind::stencil ? "fourier"
+                 : pc.kind() == Precond::Kind::fourier ? "fourier-fft" : "none");
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -133,32 +216,12 @@ int aspher_device_count(int* count) {
 
 int aspher_solver_create(int grid_size, double domain_size, double E_star, const char* backend,
                          aspher_solver** out) {
-    return guard([&] {
-        if (!out) throw std::invalid_argument("null output handle");
-        // validation order of python/bindings.cpp:122-127
-        if (grid_size <= 0) throw std::invalid_argument("grid_size must be positive");
-        if (!(domain_size > 0.0)) throw std::invalid_argument("domain_size must be positive");
-        if (!(E_star > 0.0)) throw std::invalid_argument("E_star must be positive");
-        const std::string b = backend ? backend : "";
-        Operator::Kind kind;
-        if (b == "fft") kind = Operator::Kind::free_space;
-        else if (b == "periodic") kind = Operator::Kind::periodic;
-        else if (b == "dense") kind = Operator::Kind::dense;
-        else if (b == "h2" || b == "hmatrix" || b.empty())
-            throw std::invalid_argument("backend '" + (b.empty() ? std::string("hmatrix") : b) +
-                                        "' is not part of this build (expected fft, periodic "
-                                        "or dense)");
-        else
-            throw std::invalid_argument("unknown backend: " + b +
-                                        " (expected dense, hmatrix, h2, fft, or periodic)");
-        auto s = std::make_unique<aspher_solver>();
-        s->ctx = std::make_unique<Context>();
-        s->op = std::make_shared<Operator>(kind, grid_size, domain_size, E_star, s->ctx->stream);
-        s->op->build();
-        s->backend = b;
-        s->Ns = grid_size;
-        *out = s.release();
-    });
+    return guard([&] { create_solver(grid_size, domain_size, E_star, backend, false, out); });
+}
+
+int aspher_solver_create_shared(int grid_size, double domain_size, double E_star,
+                                const char* backend, aspher_solver** out) {
+    return guard([&] { create_solver(grid_size, domain_size, E_star, backend, true, out); });
 }
 
 void aspher_solver_destroy(aspher_solver* s) {
@@ -191,7 +254,7 @@ int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int pr
         if (precision == 64) {
             s->op->apply(static_cast<const double*>(x), static_cast<double*>(y), st);
         } else if (precision == 32) {
-            s->op->build_single();
+            s->op->build_single(st);
             s->op->apply(static_cast<const float*>(x), static_cast<float*>(y), st);
         } else {
             throw std::invalid_argument("precision must be 64 or 32");
@@ -206,7 +269,7 @@ int aspher_solver_time_matvec(aspher_solver* s, const void* x, void* y, int prec
         if (!s || !x || !y || !ms || reps <= 0) throw std::invalid_argument("bad arguments");
         ASPHER_HIP(hipDeviceSynchronize());
         hipStream_t st = s->ctx->stream;
-        if (precision == 32) s->op->build_single();
+        if (precision == 32) s->op->build_single(s->ctx->stream);
         auto run = [&] {
             if (precision == 64)
                 s->op->apply(static_cast<const double*>(x), static_cast<double*>(y), st);
@@ -237,7 +300,7 @@ int aspher_solver_time_stages(aspher_solver* s, const void* x, void* y, int prec
             throw std::invalid_argument("precision must be 64 or 32");
         ASPHER_HIP(hipDeviceSynchronize());
         hipStream_t st = s->ctx->stream;
-        if (precision == 32) s->op->build_single();
+        if (precision == 32) s->op->build_single(s->ctx->stream);
         std::vector<hipEvent_t> ev(4 * (std::size_t)reps);
         for (auto& e : ev) ASPHER_HIP(hipEventCreate(&e));
         auto run = [&](hipEvent_t* e) {
@@ -578,6 +641,42 @@ int aspher_precond_apply(int grid_size, const char* kind, int radius, int precis
 
 int aspher_clear_cache(void) {
     return guard([&] { clear_operator_cache(); });
+}
+
+
+void aspher_adjoint_params_default(aspher_adjoint_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->tol = 1e-10;
+    p->max_iter = 5000;
+    std::strcpy(p->precond, "auto");
+}
+
+int aspher_solver_adjoint(aspher_solver* s, const aspher_result* res,
+                          const double* grad_pressure, const aspher_adjoint_params* prm,
+                          double* grad_gap, aspher_adjoint_info* info) {
+    return guard([&] {
+        if (!s || !grad_pressure || !grad_gap) throw std::invalid_argument("null pointer");
+        const std::int64_t N = (std::int64_t)s->Ns * s->Ns;
+        DevBuf<double> w(N), g(N);
+        hipStream_t st = s->ctx->stream;
+        ASPHER_HIP(hipMemcpyAsync(w.p, grad_pressure, sizeof(double) * N,
+                                  hipMemcpyHostToDevice, st));
+        adjoint_device(s, res, w.p, prm, g.p, info);
+        ASPHER_HIP(hipMemcpyAsync(grad_gap, g.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_solver_adjoint_device(aspher_solver* s, const aspher_result* res,
+                                 const double* grad_pressure_dev,
+                                 const aspher_adjoint_params* prm, double* grad_gap_dev,
+                                 aspher_adjoint_info* info) {
+    return guard([&] {
+        ASPHER_HIP(hipDeviceSynchronize());
+        adjoint_device(s, res, grad_pressure_dev, prm, grad_gap_dev, info);
+        ASPHER_HIP(hipStreamSynchronize(s->ctx->stream));
+    });
 }
 
 int aspher_device_malloc(int64_t bytes, void** ptr) {

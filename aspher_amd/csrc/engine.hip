@@ -76,8 +76,8 @@ void Context::read_scal() {
 
 // ── Operator ───────────────────────────────────────────────────────────────
 
-Operator::Operator(Kind kind, int Ns, double L, double E_star, hipStream_t s)
-    : kind_(kind), Ns_(Ns), L_(L), E_(E_star), stream_(s) {
+Operator::Operator(Kind kind, int Ns, double L, double E_star)
+    : kind_(kind), Ns_(Ns), L_(L), E_(E_star) {
     if (Ns <= 0) throw std::invalid_argument("grid_size must be positive");
     if (!(L > 0.0)) throw std::invalid_argument("domain_size must be positive");
     if (!(E_star > 0.0)) throw std::invalid_argument("E_star must be positive");
@@ -87,7 +87,7 @@ Operator::Operator(Kind kind, int Ns, double L, double E_star, hipStream_t s)
 
 Operator::~Operator() = default;
 
-void Operator::build() {
+void Operator::build(hipStream_t s) {
     const auto t0 = Clock::now();
     const std::int64_t N = (std::int64_t)Ns_ * Ns_;
     if (kind_ == Kind::free_space) {
@@ -96,16 +96,16 @@ void Operator::build() {
         // unique values Re/M^2 are kept (src/fft_operator.cpp:18-69).
         const int M = 2 * Ns_;
         DevBuf<double> table(N);
-        love_table(Ns_, L_, E_, table.p, stream_);
+        love_table(Ns_, L_, E_, table.p, s);
         kh64_.alloc((std::size_t)(Ns_ + 1) * (Ns_ + 1));
         {
             SpectralPlan<double> bplan(M, M, M, 0, 0);
             DevBuf<double> part((std::size_t)2 * bplan.col_blocks());
-            bplan.build_spectrum(table.p, Ns_, kh64_.p, part.p, stream_);
+            bplan.build_spectrum(table.p, Ns_, kh64_.p, part.p, s);
             std::vector<double> h(part.n);
             ASPHER_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes(), hipMemcpyDeviceToHost,
-                                      stream_));
-            ASPHER_HIP(hipStreamSynchronize(stream_));
+                                      s));
+            ASPHER_HIP(hipStreamSynchronize(s));
             double mi = 0.0, mr = 0.0;
             for (std::size_t b = 0; b < h.size() / 2; ++b) {
                 mi = std::max(mi, h[2 * b]);
@@ -121,18 +121,18 @@ void Operator::build() {
         plan64_ = std::make_unique<SpectralPlan<double>>(Ns_, Ns_, Ns_, Ns_, Ns_);
     } else {
         table_.alloc(N);
-        love_table(Ns_, L_, E_, table_.p, stream_);
+        love_table(Ns_, L_, E_, table_.p, s);
     }
-    ASPHER_HIP(hipStreamSynchronize(stream_));
+    ASPHER_HIP(hipStreamSynchronize(s));
     built_ = true;
     build_seconds = since(t0);
 }
 
-void Operator::build_single() {
+void Operator::build_single(hipStream_t s) {
     if (!built_) throw std::logic_error("FFTOperator::build_single_caches: build() not called");
     if (kind_ == Kind::free_space && !kh32_.p) {
         kh32_.alloc(kh64_.n);
-        cast_d2f((std::int64_t)kh64_.n, kh64_.p, kh32_.p, stream_);
+        cast_d2f((std::int64_t)kh64_.n, kh64_.p, kh32_.p, s);
     }
     if (kind_ != Kind::dense && !plan32_) {
         const int n = kind_ == Kind::free_space ? 2 * Ns_ : Ns_;
@@ -147,12 +147,12 @@ void Operator::release_single() {
 }
 
 template <>
-SpectralPlan<double>& Operator::plan<double>() {
+SpectralPlan<double>& Operator::plan<double>(hipStream_t) {
     return *plan64_;
 }
 template <>
-SpectralPlan<float>& Operator::plan<float>() {
-    if (!plan32_) build_single();
+SpectralPlan<float>& Operator::plan<float>(hipStream_t s) {
+    if (!plan32_) build_single(s);
     return *plan32_;
 }
 template <typename T>
@@ -162,17 +162,17 @@ void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
         const T* kh;
         if constexpr (std::is_same_v<T, double>) kh = kh64_.p;
         else {
-            if (!kh32_.p) build_single();
+            if (!kh32_.p) build_single(s);
             kh = kh32_.p;
         }
-        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::table, kh, Ns_ + 1, 0.0, s, RowIn::plain,
+        plan<T>(s).convolve(x, Ns_, y, Ns_, Symbol::table, kh, Ns_ + 1, 0.0, s, RowIn::plain,
                            RowOut::plain, nullptr, nullptr, 0, nullptr, ev);
     } else if (kind_ == Kind::periodic) {
         // q = 2 pi k/L, u_hat = 2 p_hat/(E*|q|), unnormalised pair -> 1/Ns^2
         // (src/periodic_fft_operator.cpp:11-24)
         const double pi = std::acos(-1.0);
         const double sc = L_ / (pi * E_ * static_cast<double>(Ns_) * Ns_);
-        plan<T>().convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s, RowIn::plain,
+        plan<T>(s).convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s, RowIn::plain,
                            RowOut::plain, nullptr, nullptr, 0, nullptr, ev);
     } else {
         if (ev) throw std::invalid_argument("stage timing needs a spectral backend");

@@ -114,15 +114,15 @@ struct Result {
 class Operator {
 public:
     enum class Kind { free_space, periodic, dense };
-    Operator(Kind kind, int Ns, double L, double E_star, hipStream_t s);
+    Operator(Kind kind, int Ns, double L, double E_star);
     ~Operator();
     Operator(const Operator&) = delete;
     Operator& operator=(const Operator&) = delete;
 
     // free space: FP64 Love table + FP64 padded r2c of its embedding,
     // realness check, (Ns+1)^2 real spectrum (src/fft_operator.cpp:18-69)
-    void build();
-    void build_single();    // FP32 spectrum copy + FP32 plan
+    void build(hipStream_t s);
+    void build_single(hipStream_t s);    // FP32 spectrum copy + FP32 plan
     void release_single();
     bool built() const { return built_; }
 
@@ -145,12 +145,11 @@ public:
 
 private:
     template <typename T> void apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev);
-    template <typename T> SpectralPlan<T>& plan();
+    template <typename T> SpectralPlan<T>& plan(hipStream_t s);
 
     Kind kind_;
     int Ns_;
     double L_, E_;
-    hipStream_t stream_;
     bool built_ = false;
     DevBuf<double> table_;
     DevBuf<double> kh64_;      // (Ns+1) x (Ns+1), [kx][ky]
@@ -248,6 +247,37 @@ bool stencil_for_level(const std::string& engine, bool precond, double prev_occu
 void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double p_bar,
                   double tol, int max_iter, bool use_pr, const NestedParams& np,
                   Result& res, bool reuse_operators = false);
+
+// Adjoint of a converged solve (no counterpart in the reference). For a
+// loss with dL/dp = w it returns dL/dg0 (Ns^2, zero off the contact set) and
+// dL/dp_nominal, from the transposed KKT system of the contact QP on the
+// support C = {p > 0}:
+//   [S_CC 1; 1' 0][mu; nu] = [w_C; 0],  dL/dg0 = -mu on C,
+//   dL/dP = nu, dL/dp_nominal = nu Ns^2,
+// solved by preconditioned CG on the zero-mean subspace of C with the
+// forward operator (same spectrum) and preconditioner; iterates are
+// recomputed, nothing of the forward CG is stored.
+struct AdjointOptions {
+    double tol = 1e-10;        // relative residual ||r|| / ||r0||
+    int max_iter = 5000;
+};
+
+struct AdjointInfo {
+    bool converged = false;
+    int iterations = 0;
+    long long n_contact = 0;
+    double rel_residual = 0.0;
+    double multiplier = 0.0;        // nu = dL/dP
+    double grad_p_nominal = 0.0;    // nu * Ns^2
+    long long matvec_count = 0, precond_count = 0;
+    double seconds = 0.0;
+    std::string status;             // converged | max_iterations | nonpositive_curvature
+};
+
+// p: converged pressure (device FP64), w: dL/dp (device FP64), grad: output
+// dL/dg0 (device FP64); all Ns^2 values.
+void adjoint_impl(Context& ctx, Operator& op, Precond* pc, const double* p, const double* w,
+                  double* grad, const AdjointOptions& opt, AdjointInfo& info);
 
 // Process-wide operator cache (kind, Ns, L, E) -> built operator.
 std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L, double E,

@@ -162,44 +162,29 @@ struct Shape {
 __device__ __forceinline__ int lds_ix(int p) { return p + (p >> 5); }
 constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
 
-// twiddles W^{v*idx}, v = 0..N-1 (table tw[t] = exp(-2 pi i t/len), tmask =
-// len - 1), applied to a group in the working precision: anchors w^1, w^4,
-// w^8, w^12 are read from the table, w^(4a+b) = w^(4a) w^b with w^2, w^3 from
-// products, so every twiddle carries at most four roundings
+// twiddles W^{v*idx}, v = 0..N-1, applied to a group: one FP64 table read
+// (tw[t] = exp(-2 pi i t/len)), powers as a running FP64 product (at most 15
+// roundings, ~1e-15 relative), rounded once to T. One table read per group
+// measured faster than reading every power (the table reads, not the FP64
+// arithmetic, bound the stage).
 template <typename T, int NS, bool INV>
-__device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<T>* __restrict__ tw,
-                                              int idx, int tmask) {
+__device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<double>* __restrict__ tw,
+                                              int idx, int /*tmask*/) {
     if (idx == 0) return;
-    auto rd = [&](int e) {
-        cx<T> w = tw[e & tmask];
-        if (INV) w.im = -w.im;
-        return w;
-    };
-    const cx<T> w1 = rd(idx);
-    if constexpr (NS == 2) {
-        a[1] = a[1] * w1;
-        return;
-    }
-    const cx<T> w2 = w1 * w1, w3 = w2 * w1;
-    a[1] = a[1] * w1;
-    a[2] = a[2] * w2;
-    a[3] = a[3] * w3;
-    if constexpr (NS > 4) {
+    cx<double> w = tw[idx];
+    if (INV) w.im = -w.im;
+    cx<double> wv = w;
 #pragma unroll
-        for (int h = 4; h < NS; h += 4) {
-            const cx<T> wh = rd(h * idx);
-            a[h] = a[h] * wh;
-            a[h + 1] = a[h + 1] * (wh * w1);
-            a[h + 2] = a[h + 2] * (wh * w2);
-            a[h + 3] = a[h + 3] * (wh * w3);
-        }
+    for (int v = 1; v < NS; ++v) {
+        if (v > 1) wv = wv * w;
+        a[v] = a[v] * cx<T>{static_cast<T>(wv.re), static_cast<T>(wv.im)};
     }
 }
 
 // Stage s of the forward (INV=false, DFT then twiddle) or inverse (INV=true,
 // twiddle then DFT) transform on the registers of thread th.
 template <typename T, int LOGN, int s, bool INV>
-__device__ __forceinline__ void stage(cx<T>* r, int th, const cx<T>* __restrict__ tw,
+__device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __restrict__ tw,
                                       int twmul) {
     using Sh = Shape<LOGN>;
     constexpr int Ns = Sh::N(s), Ss = Sh::S(s), Gs = Sh::G(s);
@@ -239,7 +224,7 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th) {
 
 template <typename T, int LOGN, int s>
 __device__ __forceinline__ void forward_from(cx<T>* r, cx<T>* lds, int th,
-                                             const cx<T>* __restrict__ tw, int twmul) {
+                                             const cx<double>* __restrict__ tw, int twmul) {
     stage<T, LOGN, s, false>(r, th, tw, twmul);
     if constexpr (s + 1 < Shape<LOGN>::m) {
         exchange<T, LOGN, s, s + 1>(r, lds, th);
@@ -249,7 +234,7 @@ __device__ __forceinline__ void forward_from(cx<T>* r, cx<T>* lds, int th,
 
 template <typename T, int LOGN, int s>
 __device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
-                                             const cx<T>* __restrict__ tw, int twmul) {
+                                             const cx<double>* __restrict__ tw, int twmul) {
     stage<T, LOGN, s, true>(r, th, tw, twmul);
     if constexpr (s > 0) {
         exchange<T, LOGN, s, s - 1>(r, lds, th);
@@ -261,12 +246,12 @@ __device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
 // grouping (digit-reversed); inverse: the reverse.
 template <typename T, int LOGN>
 __device__ __forceinline__ void forward(cx<T>* r, cx<T>* lds, int th,
-                                        const cx<T>* __restrict__ tw, int twmul) {
+                                        const cx<double>* __restrict__ tw, int twmul) {
     forward_from<T, LOGN, 0>(r, lds, th, tw, twmul);
 }
 template <typename T, int LOGN>
 __device__ __forceinline__ void inverse(cx<T>* r, cx<T>* lds, int th,
-                                        const cx<T>* __restrict__ tw, int twmul) {
+                                        const cx<double>* __restrict__ tw, int twmul) {
     inverse_from<T, LOGN, Shape<LOGN>::m - 1>(r, lds, th, tw, twmul);
 }
 

@@ -508,6 +508,34 @@ __device__ void run_step(int step, double* s) {
                 }
             }
             break;
+        case STEP_ADJ_INIT:
+            s[S_GMEAN] = nc > 0.0 ? s[S_A_WSUM] / nc : 0.0;
+            s[S_A_RZOLD] = 0.0;
+            s[S_A_BETA] = 0.0;
+            s[S_A_OK] = 1.0;
+            break;
+        case STEP_ADJ_DOT: {
+            const double rz = s[S_A_RZ], old = s[S_A_RZOLD];
+            s[S_A_BETA] = old != 0.0 ? rz / old : 0.0;
+            s[S_A_RZOLD] = rz;
+            break;
+        }
+        case STEP_ADJ_CURV: {
+            const double qmean = nc > 0.0 ? s[S_A_QSUM] / nc : 0.0;
+            const double den = s[S_A_QT] - qmean * s[S_A_TSUM];
+            const double rz = s[S_A_RZOLD];
+            const bool ok = den > 0.0 && isfinite(den) && isfinite(rz);
+            s[S_A_QMEAN] = qmean;
+            s[S_A_OK] = ok ? 1.0 : 0.0;
+            s[S_A_ALPHA] = ok ? rz / den : 0.0;
+            break;
+        }
+        case STEP_ADJ_RMEAN:
+            s[S_GMEAN] = nc > 0.0 ? s[S_A_RSUM] / nc : 0.0;
+            break;
+        case STEP_ADJ_FINISH:
+            s[S_A_NU] = nc > 0.0 ? s[S_A_WQSUM] / nc : 0.0;
+            break;
         default:
             break;
     }
@@ -627,6 +655,121 @@ k_prolong2(int Nc, const double* __restrict__ pc, double* __restrict__ out) {
 __global__ void __launch_bounds__(256)
 k_cast_d2f(std::int64_t N, const double* __restrict__ x, float* __restrict__ y) {
     GRID_LOOP(i, N) y[i] = (float)x[i];
+}
+
+
+// ── adjoint: linear CG on a fixed contact set ──────────────────────────────
+// Solves [S_CC 1; 1' 0][mu; nu] = [w_C; 0] (the transposed KKT system of the
+// contact QP on its converged support) as CG on the zero-mean subspace of C.
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_adj_init(std::int64_t N, const double* __restrict__ p, const T* __restrict__ w,
+           std::uint8_t* __restrict__ mask, T* __restrict__ r, T* __restrict__ x,
+           T* __restrict__ t, double* __restrict__ part) {
+    __shared__ double red[8];
+    double ws = 0.0, nc = 0.0;
+    GRID_LOOP(i, N) {
+        const bool c = p[i] > 0.0;
+        mask[i] = c ? 1 : 0;
+        const T wi = c ? w[i] : T(0);
+        r[i] = wi;
+        x[i] = T(0);
+        t[i] = T(0);
+        if (c) {
+            ws += (double)wi;
+            nc += 1.0;
+        }
+    }
+    put<RSUM>(ws, red, part, 2, 0);
+    put<RSUM>(nc, red, part, 2, 1);
+}
+
+template <typename T, bool PC>
+__global__ void __launch_bounds__(256)
+k_adj_dot(std::int64_t N, const T* __restrict__ r, const T* __restrict__ z,
+          const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
+          double* __restrict__ part) {
+    __shared__ double red[8];
+    const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
+    double rz = 0.0, rr = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        const double ri = (double)r[i];
+        const double zi = PC ? (double)(T)(z[i] - zmean) : ri;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    put<RSUM>(rz, red, part, 2, 0);
+    put<RSUM>(rr, red, part, 2, 1);
+}
+
+template <typename T, bool PC>
+__global__ void __launch_bounds__(256)
+k_adj_dir(std::int64_t N, const T* __restrict__ r, const T* __restrict__ z,
+          const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
+          T* __restrict__ t) {
+    const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
+    const T beta = static_cast<T>(scal[S_A_BETA]);
+    GRID_LOOP(i, N) {
+        const T zi = PC ? T(z[i] - zmean) : r[i];
+        t[i] = mask[i] ? T(zi + beta * t[i]) : T(0);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_adj_curv(std::int64_t N, const T* __restrict__ q, const T* __restrict__ t,
+           const std::uint8_t* __restrict__ mask, double* __restrict__ part) {
+    __shared__ double red[8];
+    double qs = 0.0, qt = 0.0, ts = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        const double qi = (double)q[i], ti = (double)t[i];
+        qs += qi;
+        qt += qi * ti;
+        ts += ti;
+    }
+    put<RSUM>(qs, red, part, 3, 0);
+    put<RSUM>(qt, red, part, 3, 1);
+    put<RSUM>(ts, red, part, 3, 2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_adj_step(std::int64_t N, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ t,
+           const T* __restrict__ q, const std::uint8_t* __restrict__ mask,
+           const double* __restrict__ scal, double* __restrict__ part) {
+    __shared__ double red[8];
+    const bool ok = scal[S_A_OK] != 0.0;
+    const T a = static_cast<T>(scal[S_A_ALPHA]), qm = static_cast<T>(scal[S_A_QMEAN]);
+    double rs = 0.0;
+    GRID_LOOP(i, N) {
+        if (!mask[i]) continue;
+        T ri = r[i];
+        if (ok) {
+            x[i] = x[i] + a * t[i];
+            ri = ri - a * (q[i] - qm);
+            r[i] = ri;
+        }
+        rs += (double)ri;
+    }
+    put<RSUM>(rs, red, part, 1, 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_adj_finish(std::int64_t N, const T* __restrict__ x, const T* __restrict__ w,
+             const T* __restrict__ q, const std::uint8_t* __restrict__ mask,
+             double* __restrict__ grad, double* __restrict__ part) {
+    __shared__ double red[8];
+    double s = 0.0;
+    GRID_LOOP(i, N) {
+        const bool c = mask[i] != 0;
+        grad[i] = c ? -(double)x[i] : 0.0;
+        if (c) s += (double)w[i] - (double)q[i];
+    }
+    put<RSUM>(s, red, part, 1, 0);
 }
 
 } // namespace
@@ -765,6 +908,43 @@ void VecOps<T>::dense(int Ns, const double* table, const T* x, T* y, hipStream_t
 
 template struct VecOps<float>;
 template struct VecOps<double>;
+
+template <typename T>
+void AdjOps<T>::init(std::int64_t N, const double* p, const T* w, std::uint8_t* mask, T* r,
+                     T* x, T* t, double* part, int nblk, hipStream_t s) {
+    L1(k_adj_init<T>, N, p, w, mask, r, x, t, part);
+}
+template <typename T>
+void AdjOps<T>::dot(std::int64_t N, const T* r, const T* z, const std::uint8_t* mask,
+                    const double* scal, double* part, int nblk, hipStream_t s) {
+    if (z) { L1((k_adj_dot<T, true>), N, r, z, mask, scal, part); }
+    else { L1((k_adj_dot<T, false>), N, r, z, mask, scal, part); }
+}
+template <typename T>
+void AdjOps<T>::dir(std::int64_t N, const T* r, const T* z, const std::uint8_t* mask,
+                    const double* scal, T* t, hipStream_t s) {
+    if (z) { L1S((k_adj_dir<T, true>), N, r, z, mask, scal, t); }
+    else { L1S((k_adj_dir<T, false>), N, r, z, mask, scal, t); }
+}
+template <typename T>
+void AdjOps<T>::curv(std::int64_t N, const T* q, const T* t, const std::uint8_t* mask,
+                     double* part, int nblk, hipStream_t s) {
+    L1(k_adj_curv<T>, N, q, t, mask, part);
+}
+template <typename T>
+void AdjOps<T>::step(std::int64_t N, T* x, T* r, const T* t, const T* q,
+                     const std::uint8_t* mask, const double* scal, double* part, int nblk,
+                     hipStream_t s) {
+    L1(k_adj_step<T>, N, x, r, t, q, mask, scal, part);
+}
+template <typename T>
+void AdjOps<T>::finish(std::int64_t N, const T* x, const T* w, const T* q,
+                       const std::uint8_t* mask, double* grad, double* part, int nblk,
+                       hipStream_t s) {
+    L1(k_adj_finish<T>, N, x, w, q, mask, grad, part);
+}
+
+template struct AdjOps<double>;
 
 template <typename T>
 void finalize(const double* part, int nblk, int nv, const int* ops, const int* dst,

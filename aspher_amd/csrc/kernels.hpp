@@ -22,12 +22,16 @@ enum Scal : int {
     S_ZSUM, S_ZMEAN,
     S_PTOT, S_GREF, S_USEPR, S_PBAR,
     S_R0, S_R1, S_R2, S_R3, S_R4, S_R5,   // generic reduction results
+    // adjoint CG on a fixed contact set
+    S_A_WSUM, S_A_RZ, S_A_RZOLD, S_A_RR, S_A_BETA, S_A_QSUM, S_A_QT, S_A_TSUM,
+    S_A_ALPHA, S_A_QMEAN, S_A_OK, S_A_RSUM, S_A_WQSUM, S_A_NU,
     S_COUNT
 };
 
 enum Step : int {
     STEP_NONE = 0, STEP_EVAL1, STEP_EVAL2, STEP_ZMEAN, STEP_CGDOT, STEP_LINE1,
     STEP_LINE2, STEP_UPDATE,
+    STEP_ADJ_INIT, STEP_ADJ_DOT, STEP_ADJ_CURV, STEP_ADJ_RMEAN, STEP_ADJ_FINISH,
 };
 
 template <typename T> struct VecOps {
@@ -93,6 +97,32 @@ template <typename T> struct VecOps {
     static void masked_sub(std::int64_t N, T* y, const std::uint8_t* mask,
                            const double* scal, int idx, hipStream_t s);
     static void dense(int Ns, const double* table, const T* x, T* y, hipStream_t s);
+};
+
+// Passes of the adjoint solve: the linear CG for [S_CC 1; 1' 0][mu; nu] =
+// [w_C; 0] on the zero-mean subspace of the contact set C = {p > 0}.
+template <typename T> struct AdjOps {
+    // mask = p > 0; r = mask ? w : 0; x = t = 0; partials {sum_C w, n_C}
+    static void init(std::int64_t N, const double* p, const T* w, std::uint8_t* mask, T* r,
+                     T* x, T* t, double* part, int nblk, hipStream_t s);
+    // partials {sum_C r (z - zmean), sum_C r^2}; z == nullptr means z = r
+    static void dot(std::int64_t N, const T* r, const T* z, const std::uint8_t* mask,
+                    const double* scal, double* part, int nblk, hipStream_t s);
+    // t = mask ? (z - zmean) + beta t : 0 (z == nullptr: z = r)
+    static void dir(std::int64_t N, const T* r, const T* z, const std::uint8_t* mask,
+                    const double* scal, T* t, hipStream_t s);
+    // partials {sum_C q, sum_C q t, sum_C t}
+    static void curv(std::int64_t N, const T* q, const T* t, const std::uint8_t* mask,
+                     double* part, int nblk, hipStream_t s);
+    // predicated on S_A_OK: x += alpha t, r -= alpha (q - qmean) on C;
+    // partial {sum_C r}
+    static void step(std::int64_t N, T* x, T* r, const T* t, const T* q,
+                     const std::uint8_t* mask, const double* scal, double* part, int nblk,
+                     hipStream_t s);
+    // grad = mask ? -x : 0; partial {sum_C (w - q)}
+    static void finish(std::int64_t N, const T* x, const T* w, const T* q,
+                       const std::uint8_t* mask, double* grad, double* part, int nblk,
+                       hipStream_t s);
 };
 
 // Reduce `nv` partial columns (ops: RSUM/RMIN/RMAX) into scal[dst[v]] and run

@@ -449,8 +449,8 @@ std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L,
     auto key = std::make_tuple((int)kind, Ns, L, E);
     auto it = g_cache.find(key);
     if (it != g_cache.end()) return it->second;
-    auto op = std::make_shared<Operator>(kind, Ns, L, E, s);
-    op->build();
+    auto op = std::make_shared<Operator>(kind, Ns, L, E);
+    op->build(s);
     g_cache[key] = op;
     return op;
 }
@@ -595,8 +595,8 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
         if (reuse_operators) {
             op = cached_operator(kind, n, L, E_star, s);
         } else {
-            op = std::make_shared<Operator>(kind, n, L, E_star, s);
-            op->build();
+            op = std::make_shared<Operator>(kind, n, L, E_star);
+            op->build(s);
         }
         const int r_lvl = std::min({np.precond_radius, std::max(1, n / 4), np.leaf_side});
         const std::string engine =
@@ -636,7 +636,7 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
 
         Result lres;
         if (level_float) {
-            op->build_single();
+            op->build_single(s);
             // the datum is removed inside the cast, before any rounding
             DevBuf<float> g0f(nn);
             VecOps<float>::from_double(nn, glvl, datum, g0f.p, s);

@@ -376,7 +376,7 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
         row_batch_ = B;
         col_width_ = W;
         pitchT_ = (std::max(rows_in, 1) + 31) / 32 * 32 + 32;
-        tw_ = std::make_unique<Twiddles<T>>(n);
+        tw64_ = std::make_unique<Twiddles<double>>(n);
         a_rows_ = static_cast<std::size_t>(std::max(rows_out, 1));
         ASPHER_HIP(hipMalloc(&A_, cs * a_rows_ * pitch));
         ASPHER_HIP(hipMalloc(&AT_, cs * (std::size_t)nh * pitchT_));
@@ -415,8 +415,9 @@ SpectralPlan<T>::~SpectralPlan() {
 
 template <typename T>
 std::size_t SpectralPlan<T>::workspace_bytes() const {
-    return sizeof(cx<T>) * a_rows_ * pitch + sizeof(cx<T>) * n +
-           (rb_ ? sizeof(cx<T>) * (std::size_t)nh * pitchT_ : 0);
+    return sizeof(cx<T>) * a_rows_ * pitch +
+           (rb_ ? sizeof(cx<double>) * n + sizeof(cx<T>) * (std::size_t)nh * pitchT_
+                : sizeof(cx<T>) * n);
 }
 
 template <typename T>
@@ -546,7 +547,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         f.AT = AT_;
         f.pitchT = pitchT_;
         f.B = row_batch_;
-        f.tw = tw_->dev;
+        f.tw = tw64_->dev;
         f.mode = in_mode == RowIn::masked ? 1 : 0;
         f.mask = mask;
         f.scal = scal;
@@ -609,7 +610,7 @@ void SpectralPlan<T>::build_spectrum(const double* table, int Ns, T* kout, doubl
         f.AT = AT_;
         f.pitchT = pitchT_;
         f.B = row_batch_;
-        f.tw = tw_->dev;
+        f.tw = tw64_->dev;
         f.mode = 2;
         f.table = table;
         f.Ns_embed = Ns;

@@ -106,10 +106,11 @@ private:
     std::size_t a_rows_ = 0;
     bool pow2_ = false;
     // register-blocked path (powers of two, 32 <= n <= 8192): transposed
-    // half spectrum AT (nh columns of pitchT)
+    // half spectrum AT (nh columns of pitchT) + FP64 twiddles
     bool rb_ = false;
     cx<T>* AT_ = nullptr;
     int pitchT_ = 0;
+    std::unique_ptr<Twiddles<double>> tw64_;
     int logL_row_ = 0, logn_ = 0;
     int row_batch_ = 1, col_width_ = 1;
     bool xcd_remap_ = true;

@@ -32,7 +32,7 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
 template <typename T, int LOGL, int IN>
 __global__ void __launch_bounds__(512)
 k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
-              cx<T>* __restrict__ AT, int pitchT, int B, const cx<T>* __restrict__ tw,
+              cx<T>* __restrict__ AT, int pitchT, int B, const cx<double>* __restrict__ tw,
               const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
               int gmean_idx, const double* __restrict__ table, int Ns_embed, int remap) {
     using Sh = rb::Shape<LOGL>;
@@ -93,8 +93,8 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
         const cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};
         // W^k = W^(k - k%64) W^(k%64): both factors from a small, cache-hot
         // subset of the table
-        const cx<T> w = tw[k & ~63] * tw[k & 63];
-        const cx<T> X = E + w * O;
+        const cx<double> wd = tw[k & ~63] * tw[k & 63];
+        const cx<T> X = E + cx<T>{static_cast<T>(wd.re), static_cast<T>(wd.im)} * O;
         AT[(size_t)k * pitchT + blk * B + bb] = X;
     }
 }
@@ -121,7 +121,7 @@ __device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restric
 template <typename T, int LOGN, int SYM>
 __global__ void __launch_bounds__(512)
 k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int pitch,
-          int ncols, int rows_in, int rows_out, int W, const cx<T>* __restrict__ tw,
+          int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
           const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
           double* __restrict__ part, int remap) {
     using Sh = rb::Shape<LOGN>;
@@ -191,7 +191,7 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
 template <typename T, int LOGL, int OUT>
 __global__ void __launch_bounds__(512)
 k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
-              const cx<T>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
+              const cx<double>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
               const std::uint8_t* __restrict__ mask, double* __restrict__ part, int remap) {
     using Sh = rb::Shape<LOGL>;
     constexpr int L = Sh::n, TPR = Sh::threads;
@@ -227,8 +227,8 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
             }
             Xc = conjg(Xc);
             const cx<T> S = Xk + Xc;
-            const cx<T> w = tw[k & ~63] * tw[k & 63];
-            const cx<T> D = (Xk - Xc) * conjg(w);
+            const cx<double> wd = tw[k & ~63] * tw[k & 63];
+            const cx<T> D = (Xk - Xc) * cx<T>{static_cast<T>(wd.re), static_cast<T>(-wd.im)};
             r[q * Nl + v] = {S.re - D.im, S.im + D.re};
         }
     rb::inverse<T, LOGL>(r, lds, th, tw, 2);

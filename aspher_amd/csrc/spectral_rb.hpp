@@ -14,7 +14,7 @@ struct RbRowsFwd {
     cx<T>* AT = nullptr;        // transposed half spectrum, AT[kx*pitchT + iy]
     int pitchT = 0;
     int B = 1;                  // rows per workgroup
-    const cx<T>* tw = nullptr;     // exp(-2 pi i t/n), n = 2L
+    const cx<double>* tw = nullptr;   // exp(-2 pi i t/n), n = 2L
     int mode = 0;               // 0 plain, 1 masked (mask ? x - gmean : 0), 2 Love embedding
     const std::uint8_t* mask = nullptr;
     const double* scal = nullptr;
@@ -32,7 +32,7 @@ struct RbCols {
     int pitch = 0;
     int ncols = 0, rows_in = 0, rows_out = 0;
     int W = 1;                  // columns per workgroup
-    const cx<T>* tw = nullptr;     // exp(-2 pi i t/n)
+    const cx<double>* tw = nullptr;   // exp(-2 pi i t/n)
     int sym = 0;                // 0 table, 1 periodic, 2 precond, 3 build
     const T* ktab = nullptr;
     int kp = 0;
@@ -47,7 +47,7 @@ struct RbRowsInv {
     const cx<T>* BR = nullptr;
     int pitch = 0, nrows = 0;
     int B = 1;
-    const cx<T>* tw = nullptr;     // exp(-2 pi i t/n), n = 2L
+    const cx<double>* tw = nullptr;   // exp(-2 pi i t/n), n = 2L
     T* y = nullptr;
     int y_pitch = 0, ncols = 0;
     int mode = 0;               // 0 plain, 1 masked with per-block sum

@@ -202,6 +202,50 @@ int aspher_precond_apply(int grid_size, const char* kind, int radius, int precis
 /* Drop the process-wide operator cache. */
 int aspher_clear_cache(void);
 
+/* Same as aspher_solver_create, but the operator (Love table + kernel
+ * spectrum) comes from the process-wide cache that solve_nested fills when
+ * it reuses operators, so a later adjoint runs on the very spectrum of the
+ * forward solve. No reference counterpart (the reference rebuilds per call). */
+int aspher_solver_create_shared(int grid_size, double domain_size, double E_star,
+                                const char* backend, aspher_solver** out);
+
+/* Adjoint of a converged solve: the backward pass of p(gap, p_nominal).
+ * No counterpart in the reference (it has no derivative of its solve); it
+ * differentiates the KKT system of the QP that solve_contact solves
+ * (src/contact_solver.cpp:150-587): for dL/dp = grad_pressure on the support
+ * C = {p > 0},
+ *     [S_CC 1; 1' 0][mu; nu] = [grad_pressure_C; 0],
+ *     grad_gap = -mu on C (0 off C),  dL/dp_nominal = nu Ns^2,
+ * solved by preconditioned CG on the support with the solver's operator. */
+typedef struct {
+    double tol;             /* relative residual of the adjoint CG; default 1e-10 */
+    int max_iter;           /* default 5000 */
+    char precond[16];       /* "auto" (default), "none", "fourier" (13-tap
+                               stencil), "fourier-fft" */
+} aspher_adjoint_params;
+
+typedef struct {
+    int converged, iterations;
+    int64_t n_contact, matvec_count, precond_count;
+    double rel_residual;
+    double multiplier;      /* nu = dL/dP, P = p_nominal Ns^2 */
+    double grad_p_nominal;  /* dL/dp_nominal */
+    double seconds;
+    char status[32];        /* converged | max_iterations | nonpositive_curvature | nonfinite */
+    char precond[16];       /* preconditioner used */
+} aspher_adjoint_info;
+
+void aspher_adjoint_params_default(aspher_adjoint_params* p);
+/* host arrays of Ns^2 doubles; `res` must come from a solve on the same grid */
+int aspher_solver_adjoint(aspher_solver* s, const aspher_result* res,
+                          const double* grad_pressure, const aspher_adjoint_params* prm,
+                          double* grad_gap, aspher_adjoint_info* info);
+/* same on device FP64 buffers */
+int aspher_solver_adjoint_device(aspher_solver* s, const aspher_result* res,
+                                 const double* grad_pressure_dev,
+                                 const aspher_adjoint_params* prm, double* grad_gap_dev,
+                                 aspher_adjoint_info* info);
+
 /* Device buffers for callers of the *_device entry points that have no
  * allocator of their own (tests, benchmarks). No reference counterpart: the
  * reference is host-only. */

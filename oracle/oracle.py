@@ -26,6 +26,9 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
   solve_contact            manufactured-KKT exact p*, support enumeration,
                            Westergaard closed form, FW bound identities
   solve_nested             nested == single level, datum invariance
+  kkt_gradient             central finite differences through the exact
+                           support-enumeration solver and solve_contact
+  circulant_matrix         equals the operator's matvec on random vectors
 """
 from __future__ import annotations
 
@@ -875,3 +878,47 @@ def spd_kernel(n, decay=1.0):
     d = np.abs(i[:, None] - i[None, :]).astype(np.float64)
     S = np.exp(-decay * d) / (1.0 + 0.25 * d) + 0.1 * np.eye(n)
     return 0.5 * (S + S.T)
+
+
+# ── adjoint of the contact solve (no counterpart in the reference) ─────────
+#
+# The reference solves min 1/2 p'Sp + g0'p s.t. p >= 0, sum p = P
+# (doc/theory/pcg.tex:380-538; KKT residuals as in tests/contact_oracle.hpp:35-61)
+# but has no derivative of its solution. On a strictly complementary solution
+# with contact set C the KKT system S_CC p_C + g0_C = alpha 1, 1'p_C = P is
+# linear, so for a loss with dL/dp = w:
+#     [S_CC 1; 1' 0] [mu; nu] = [w_C; 0],   dL/dg0 = -mu on C (0 off C),
+#     dL/dP = nu   (dL/dp_nominal = nu * N).
+# Pinned by central finite differences through the exact support-enumeration
+# solver and through solve_contact (tests/test_oracle.py).
+
+def kkt_gradient(S, p, grad_p, contact=None):
+    """Dense direct solve of the adjoint KKT system. Returns (dL/dg0, dL/dP)."""
+    p = np.asarray(p, dtype=np.float64).ravel()
+    w = np.asarray(grad_p, dtype=np.float64).ravel()
+    c = (p > 0) if contact is None else np.asarray(contact, dtype=bool).ravel()
+    idx = np.nonzero(c)[0]
+    k = idx.size
+    K = np.zeros((k + 1, k + 1))
+    K[:k, :k] = S[np.ix_(idx, idx)]
+    K[:k, k] = 1.0
+    K[k, :k] = 1.0
+    rhs = np.zeros(k + 1)
+    rhs[:k] = w[idx]
+    sol = np.linalg.solve(K, rhs)
+    g = np.zeros(p.size)
+    g[idx] = -sol[:k]
+    return g, float(sol[k])
+
+
+def circulant_matrix(matvec, Ns):
+    """Dense matrix of a translation-invariant periodic operator on the
+    Ns x Ns torus from its response to a unit impulse."""
+    e = np.zeros(Ns * Ns)
+    e[0] = 1.0
+    c0 = np.asarray(matvec(e), dtype=np.float64).reshape(Ns, Ns)
+    S = np.empty((Ns * Ns, Ns * Ns))
+    for jy in range(Ns):
+        for jx in range(Ns):
+            S[:, jy * Ns + jx] = np.roll(np.roll(c0, jy, axis=0), jx, axis=1).ravel()
+    return S

@@ -389,3 +389,81 @@ def test_nested_matches_single_level_and_datum():
         assert r.status == "converged"
         assert abs(r.approach - base.approach - shift) < 1e-6 * shift
         assert np.linalg.norm(r.pressure - base.pressure) / np.linalg.norm(base.pressure) < 1e-5
+
+
+# ── adjoint: kkt_gradient against central finite differences ───────────────
+
+def _loss(p, pt):
+    return 0.5 * float(((p - pt) ** 2).sum())
+
+
+def test_kkt_gradient_vs_exact_enumeration_fd():
+    """Exact solutions (support enumeration) make the finite differences
+    accurate to O(eps^2): gradient w.r.t. every g0 entry and P."""
+    n = 10
+    S = O.spd_kernel(n, 0.7)
+    rng = np.random.default_rng(5)
+    p_star = np.where(np.arange(n) % 3 != 1, rng.uniform(0.5, 1.5, n), 0.0)
+    g_star = np.where(p_star > 0, 0.0, rng.uniform(0.2, 0.6, n))
+    g0, P = O.manufacture(S, p_star, g_star, 0.3)
+    _, p, _ = O.enumerate_qp(S, g0, P)
+    np.testing.assert_allclose(p, p_star, atol=1e-10)
+    pt = rng.uniform(0.0, 1.0, n)
+    gg, gP = O.kkt_gradient(S, p, p - pt)
+    eps = 1e-6
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = eps
+        fd = (_loss(O.enumerate_qp(S, g0 + e, P)[1], pt) -
+              _loss(O.enumerate_qp(S, g0 - e, P)[1], pt)) / (2 * eps)
+        assert abs(fd - gg[i]) <= 1e-7 * (1 + abs(gg[i])), (i, fd, gg[i])
+        if p[i] == 0:
+            assert gg[i] == 0.0
+    fdP = (_loss(O.enumerate_qp(S, g0, P + eps)[1], pt) -
+           _loss(O.enumerate_qp(S, g0, P - eps)[1], pt)) / (2 * eps)
+    assert abs(fdP - gP) <= 1e-7 * (1 + abs(gP))
+    # a plausible sign slip (dL/dg0 = +mu) fails
+    assert np.abs(gg).max() > 1e-3
+
+
+def test_kkt_gradient_love_operator_fd():
+    """Love dense operator at Ns=8 through the oracle's PK solver."""
+    Ns = 8
+    S = O.dense_matrix(Ns)
+    x = (np.arange(Ns) + 0.5) / Ns - 0.5
+    g0 = (x[:, None] ** 2 + 0.5 * x[None, :] ** 2).ravel()
+    g0 = g0 + 0.01 * np.cos(7 * np.arange(Ns * Ns))
+    pbar = 0.05
+    opt = O.SolveOptions(tol=1e-15, max_iter=20000)
+    mv = lambda v, dtype=np.float64: S @ v  # noqa: E731
+    r = O.solve_contact(mv, g0, pbar, opt)
+    p = r.pressure
+    assert 0 < (p > 0).sum() < Ns * Ns
+    pt = np.roll(p, 3)
+    gg, gP = O.kkt_gradient(S, p, p - pt)
+    eps = 1e-6
+    v = S @ p + g0
+    gap = v - v[p > 0].mean()
+    interior = np.nonzero(p > 0.2 * p.max())[0][:6]
+    outside = np.nonzero(gap > 0.2 * gap.max())[0][:3]
+    for i in list(interior) + list(outside):
+        e = np.zeros(Ns * Ns)
+        e[i] = eps
+        lp = _loss(O.solve_contact(mv, g0 + e, pbar, opt, p_init=p.copy()).pressure, pt)
+        lm = _loss(O.solve_contact(mv, g0 - e, pbar, opt, p_init=p.copy()).pressure, pt)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - gg[i]) <= 1e-5 * max(1.0, np.abs(gg).max()), (i, fd, gg[i])
+    N = Ns * Ns
+    lp = _loss(O.solve_contact(mv, g0, pbar + eps, opt, p_init=p.copy()).pressure, pt)
+    lm = _loss(O.solve_contact(mv, g0, pbar - eps, opt, p_init=p.copy()).pressure, pt)
+    fdp = (lp - lm) / (2 * eps)
+    assert abs(fdp - gP * N) <= 1e-5 * max(1.0, abs(gP * N))
+
+
+def test_circulant_matrix_matches_periodic_operator():
+    Ns = 8
+    op = O.PeriodicOperator(Ns, 1.0, 1.0)
+    S = O.circulant_matrix(op.matvec, Ns)
+    x = np.random.default_rng(2).random(Ns * Ns)
+    np.testing.assert_allclose(S @ x, op.matvec(x), rtol=0, atol=1e-14 * np.abs(S @ x).max())
+    np.testing.assert_allclose(S, S.T, atol=1e-15 * np.abs(S).max())
