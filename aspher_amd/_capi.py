@@ -27,7 +27,8 @@ EXPORTS = (
     "aspher_precond_apply", "aspher_clear_cache", "aspher_device_malloc",
     "aspher_device_free", "aspher_memcpy_htod", "aspher_memcpy_dtoh",
     "aspher_solver_create_shared", "aspher_adjoint_params_default",
-    "aspher_solver_adjoint", "aspher_solver_adjoint_device",
+    "aspher_solver_adjoint", "aspher_solver_adjoint_device", "aspher_copy_device",
+    "aspher_ensemble_params_default", "aspher_solve_ensemble", "aspher_solve_ensemble_device",
 )
 
 
@@ -96,6 +97,21 @@ class AdjointInfo(C.Structure):
                 ("status", C.c_char * 32), ("precond", C.c_char * 16)]
 
 
+class EnsembleParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("use_pr", C.c_int),
+                ("precision", C.c_char * 16), ("precond", C.c_char * 16),
+                ("check_every", C.c_int), ("use_graph", C.c_int), ("autotune", C.c_int),
+                ("keep_best", C.c_int), ("stall_limit", C.c_int), ("light_result", C.c_int)]
+
+
+class EnsembleStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("setup_seconds", C.c_double),
+                ("loop_seconds", C.c_double), ("finish_seconds", C.c_double),
+                ("tune_seconds", C.c_double), ("groups", C.c_int), ("replays", C.c_int),
+                ("polls", C.c_int), ("host_fallbacks", C.c_int),
+                ("device_iterations", C.c_int), ("graph", C.c_int)]
+
+
 _lib = None
 
 
@@ -137,6 +153,14 @@ def _declare(L):
         "aspher_memcpy_htod": (i, [vp, vp, C.c_int64]),
         "aspher_memcpy_dtoh": (i, [vp, vp, C.c_int64]),
         "aspher_solver_create_shared": (i, [i, d, d, s, C.POINTER(vp)]),
+        "aspher_copy_device": (i, [vp, vp, C.c_int64]),
+        "aspher_ensemble_params_default": (None, [C.POINTER(EnsembleParams)]),
+        "aspher_solve_ensemble": (i, [i, ip, ip, dp, dp, dp, C.POINTER(dp),
+                                      C.POINTER(EnsembleParams), C.POINTER(vp),
+                                      C.POINTER(EnsembleStats), dp]),
+        "aspher_solve_ensemble_device": (i, [i, ip, ip, dp, dp, dp, C.POINTER(vp),
+                                             C.POINTER(EnsembleParams), C.POINTER(vp),
+                                             C.POINTER(EnsembleStats), dp]),
         "aspher_adjoint_params_default": (None, [C.POINTER(AdjointParams)]),
         "aspher_solver_adjoint": (i, [vp, vp, dp, C.POINTER(AdjointParams), dp,
                                       C.POINTER(AdjointInfo)]),

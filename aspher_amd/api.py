@@ -392,6 +392,79 @@ def solve_nested_device(grid_size, gap_ptr: int, p_nominal, tol=1e-8, max_iter=2
     return ContactResult(out.value)
 
 
+_BACKEND_CODE = {"fft": 0, "periodic": 1}
+
+
+def _ensemble_call(problems, on_device, tol, max_iter, use_pr, precision, precond,
+                   check_every, use_graph, autotune, keep_best, stall_limit, light_result):
+    n = len(problems)
+    if n == 0:
+        raise ValueError("solve_ensemble: no problems")
+    kinds, sizes, Ls, Es, pns, keep = [], [], [], [], [], []
+    gaps = (C.POINTER(C.c_double) * n)() if not on_device else (C.c_void_p * n)()
+    for i, pr in enumerate(problems):
+        b = pr.get("backend", "fft")
+        if b not in _BACKEND_CODE:
+            raise ValueError("ensemble backend must be 'fft' or 'periodic', got %r" % (b,))
+        Ns = int(pr["grid_size"])
+        kinds.append(_BACKEND_CODE[b])
+        sizes.append(Ns)
+        Ls.append(float(pr.get("domain_size", 1.0)))
+        Es.append(float(pr.get("E_star", 1.0)))
+        pns.append(float(pr["p_nominal"]))
+        if on_device:
+            gaps[i] = C.c_void_p(int(pr["gap_ptr"]))
+        else:
+            g = _flat(pr["gap"], Ns * Ns)
+            keep.append(g)
+            gaps[i] = dptr(g)
+    prm = _capi.EnsembleParams()
+    lib().aspher_ensemble_params_default(C.byref(prm))
+    prm.tol, prm.max_iter, prm.use_pr = float(tol), int(max_iter), int(bool(use_pr))
+    prm.precision, prm.precond = str(precision).encode(), str(precond).encode()
+    prm.check_every, prm.use_graph = int(check_every), int(bool(use_graph))
+    prm.autotune, prm.keep_best = int(bool(autotune)), int(bool(keep_best))
+    prm.stall_limit, prm.light_result = int(stall_limit), int(bool(light_result))
+    ia = lambda v: (C.c_int * n)(*v)          # noqa: E731
+    da = lambda v: (C.c_double * n)(*v)       # noqa: E731
+    res = (C.c_void_p * n)()
+    st = _capi.EnsembleStats()
+    secs = (C.c_double * n)()
+    fn = lib().aspher_solve_ensemble_device if on_device else lib().aspher_solve_ensemble
+    check(fn(n, ia(kinds), ia(sizes), da(Ls), da(Es), da(pns), gaps, C.byref(prm), res,
+             C.byref(st), secs))
+    results = [ContactResult(res[i]) for i in range(n)]
+    stats = {f[0]: getattr(st, f[0]) for f in _capi.EnsembleStats._fields_}
+    stats["graph"] = bool(stats["graph"])
+    stats["instance_seconds"] = [secs[i] for i in range(n)]
+    return results, stats
+
+
+def solve_ensemble(problems, tol=1e-5, max_iter=5000, use_pr=True, precision="float",
+                   precond="auto", check_every=8, use_graph=True, autotune=True,
+                   keep_best=True, stall_limit=200, light_result=False):
+    """Many independent certified solves advanced together on the GPU (no
+    reference counterpart; each instance is ContactSolver(backend).solve).
+
+    ``problems``: list of dicts with ``grid_size``, ``gap`` ((N,) or (Ns, Ns)),
+    ``p_nominal`` and optionally ``backend`` ("fft" | "periodic"),
+    ``domain_size``, ``E_star``. Returns (list of ContactResult in input
+    order, stats dict). FP32 ensembles certify at max(tol, 2e-6)."""
+    return _ensemble_call(problems, False, tol, max_iter, use_pr, precision, precond,
+                          check_every, use_graph, autotune, keep_best, stall_limit,
+                          light_result)
+
+
+def solve_ensemble_device(problems, tol=1e-5, max_iter=5000, use_pr=True, precision="float",
+                          precond="auto", check_every=8, use_graph=True, autotune=True,
+                          keep_best=True, stall_limit=200, light_result=False):
+    """solve_ensemble with each gap already resident on the device: problems
+    carry ``gap_ptr`` (FP64 device address) instead of ``gap``."""
+    return _ensemble_call(problems, True, tol, max_iter, use_pr, precision, precond,
+                          check_every, use_graph, autotune, keep_best, stall_limit,
+                          light_result)
+
+
 def configure_allocator(mmap_threshold_bytes=128 * 1024, trim_threshold_bytes=128 * 1024):
     """Host allocator policy (reference: glibc mallopt). Device memory is
     managed by the library; nothing is changed, returns False."""

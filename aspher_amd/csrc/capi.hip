@@ -2,6 +2,7 @@
 // plus a thread-local message; device state stays behind opaque handles.
 #include "../../include/aspher_hip.h"
 #include "engine.hpp"
+#include "ensemble.hpp"
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -10,6 +11,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 using namespace aspher;
 
@@ -196,6 +198,78 @@ void adjoint_device(aspher_solver* s, const aspher_result* res, const double* w_
                  pc.kind() == Precond::Kind::stencil ? "fourier"
                  : pc.kind() == Precond::Kind::fourier ? "fourier-fft" : "none");
     }
+}
+
+
+void ensemble_run(int n, const int* backend, const int* grid_size, const double* L,
+                  const double* E, const double* pn, const double* const* gaps, bool on_device,
+                  const aspher_ensemble_params* prm, aspher_result** results,
+                  aspher_ensemble_stats* stats, double* inst_seconds) {
+    if (n <= 0 || !backend || !grid_size || !pn || !gaps || !results)
+        throw std::invalid_argument("solve_ensemble: null argument or n <= 0");
+    EnsembleOptions o;
+    if (prm) {
+        o.tol = prm->tol;
+        o.max_iter = prm->max_iter;
+        o.use_pr = prm->use_pr != 0;
+        o.precision = prm->precision;
+        o.precond = prm->precond;
+        o.check_every = prm->check_every;
+        o.use_graph = prm->use_graph != 0;
+        o.autotune = prm->autotune != 0;
+        o.keep_best = prm->keep_best != 0;
+        o.stall_limit = prm->stall_limit;
+        o.light = prm->light_result != 0;
+    }
+    std::vector<EnsembleInstance> inst(n);
+    std::vector<DevBuf<double>> staged;
+    for (int i = 0; i < n; ++i) {
+        auto& e = inst[i];
+        e.kind = backend[i];
+        e.Ns = grid_size[i];
+        e.L = L ? L[i] : 1.0;
+        e.E_star = E ? E[i] : 1.0;
+        e.p_bar = pn[i];
+        if (!gaps[i]) throw std::invalid_argument("solve_ensemble: null gap");
+        if (e.Ns <= 0) throw std::invalid_argument("solve_ensemble: grid_size must be positive");
+    }
+    if (on_device) {
+        ASPHER_HIP(hipDeviceSynchronize());
+        for (int i = 0; i < n; ++i) inst[i].gap = gaps[i];
+    } else {
+        staged.resize(n);
+        for (int i = 0; i < n; ++i) {
+            const std::size_t N = (std::size_t)inst[i].Ns * inst[i].Ns;
+            staged[i].alloc(N);
+            ASPHER_HIP(hipMemcpy(staged[i].p, gaps[i], sizeof(double) * N,
+                                 hipMemcpyHostToDevice));
+            inst[i].gap = staged[i].p;
+        }
+    }
+    std::vector<Result> res;
+    EnsembleStats st;
+    solve_ensemble(inst, o, res, st);
+    for (int i = 0; i < n; ++i) {
+        auto r = std::make_unique<aspher_result>();
+        r->r = std::move(res[i]);
+        results[i] = r.release();
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->seconds = st.seconds;
+        stats->setup_seconds = st.setup_seconds;
+        stats->loop_seconds = st.loop_seconds;
+        stats->finish_seconds = st.finish_seconds;
+        stats->tune_seconds = st.tune_seconds;
+        stats->groups = st.groups;
+        stats->replays = st.replays;
+        stats->polls = st.polls;
+        stats->host_fallbacks = st.host_fallbacks;
+        stats->device_iterations = st.device_iterations;
+        stats->graph = st.graph ? 1 : 0;
+    }
+    if (inst_seconds)
+        for (int i = 0; i < n; ++i) inst_seconds[i] = st.instance_seconds[i];
 }
 
 } // namespace
@@ -679,6 +753,46 @@ int aspher_solver_adjoint_device(aspher_solver* s, const aspher_result* res,
     });
 }
 
+
+void aspher_ensemble_params_default(aspher_ensemble_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->tol = 1e-5;
+    p->max_iter = 5000;
+    p->use_pr = 1;
+    std::strcpy(p->precision, "float");
+    std::strcpy(p->precond, "auto");
+    p->check_every = 8;
+    p->use_graph = 1;
+    p->autotune = 1;
+    p->keep_best = 1;
+    p->stall_limit = 200;
+    p->light_result = 0;
+}
+
+int aspher_solve_ensemble(int n, const int* backend, const int* grid_size,
+                          const double* domain_size, const double* E_star,
+                          const double* p_nominal, const double* const* gaps,
+                          const aspher_ensemble_params* params, aspher_result** results,
+                          aspher_ensemble_stats* stats, double* instance_seconds) {
+    return guard([&] {
+        ensemble_run(n, backend, grid_size, domain_size, E_star, p_nominal, gaps, false,
+                     params, results, stats, instance_seconds);
+    });
+}
+
+int aspher_solve_ensemble_device(int n, const int* backend, const int* grid_size,
+                                 const double* domain_size, const double* E_star,
+                                 const double* p_nominal, const double* const* gaps_dev,
+                                 const aspher_ensemble_params* params,
+                                 aspher_result** results, aspher_ensemble_stats* stats,
+                                 double* instance_seconds) {
+    return guard([&] {
+        ensemble_run(n, backend, grid_size, domain_size, E_star, p_nominal, gaps_dev, true,
+                     params, results, stats, instance_seconds);
+    });
+}
+
 int aspher_device_malloc(int64_t bytes, void** ptr) {
     return guard([&] {
         if (!ptr || bytes <= 0) throw std::invalid_argument("bad allocation request");
@@ -696,6 +810,15 @@ int aspher_memcpy_htod(void* dst, const void* src, int64_t bytes) {
     return guard([&] {
         if (!dst || !src || bytes < 0) throw std::invalid_argument("bad copy");
         ASPHER_HIP(hipMemcpy(dst, src, (std::size_t)bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int aspher_copy_device(void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        if (!dst || !src || bytes < 0 || bytes % 16)
+            throw std::invalid_argument("bad copy (bytes must be a multiple of 16)");
+        copy16(bytes, src, dst, nullptr);
+        ASPHER_HIP(hipDeviceSynchronize());
     });
 }
 

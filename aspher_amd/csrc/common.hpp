@@ -71,4 +71,32 @@ inline int vec_blocks(std::int64_t n) {
     return static_cast<int>(b);
 }
 
+// ── ensembles: many independent solves in one launch ────────────────────
+// Vectors of all instances are concatenated (instance i at element offset
+// off[i]); scalar blocks sit at scal + i*sstride, partial blocks at
+// part + i*pstride. An ensemble launch takes one work item per block: a
+// chunk [begin, end) of one instance's elements, whose block reduction goes
+// to partial row `row` of that instance. Instances whose activity flag
+// (scalar `gate`) is 0 are skipped. The default Ens is a single solve:
+// grid-stride loops over one vector, exactly as without an ensemble.
+constexpr int kChunk = 4096;   // elements per work item (16 per thread)
+
+struct WorkItem {
+    std::int64_t begin, end;
+    int inst, row;
+};
+
+struct Ens {
+    const WorkItem* work = nullptr;   // null: single solve
+    int nwork = 0;
+    const std::int64_t* off = nullptr;
+    int sstride = 0;
+    std::int64_t pstride = 0;
+    const double* gscal = nullptr;
+    int gate = -1;
+    const int* ns = nullptr;          // per-instance grid side (tile work items)
+    int ninst = 1;                    // instances (finalize grid)
+    const int* rows = nullptr;        // per-instance partial rows (finalize)
+};
+
 } // namespace aspher

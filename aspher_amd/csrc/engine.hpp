@@ -142,6 +142,12 @@ public:
     double build_seconds = 0.0;
     // device Love table (dense kind; free space keeps it only if asked)
     const double* table() const { return table_.p; }
+    // free-space kernel spectrum, (Ns+1)^2 [kx][ky] (FP32 copy made on demand)
+    const double* spectrum64() const { return kh64_.p; }
+    const float* spectrum32(hipStream_t s) {
+        if (!kh32_.p) build_single(s);
+        return kh32_.p;
+    }
 
 private:
     template <typename T> void apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev);
@@ -175,6 +181,11 @@ public:
     const std::vector<int>& tap_dx() const { return dx_; }
     const std::vector<int>& tap_dy() const { return dy_; }
     const std::vector<double>& tap_w() const { return w_; }
+    // device tap table (dx, dy interleaved) and weights of the stencil kind
+    const int* taps_dev() const { return tapd_.p; }
+    const double* weights_dev() const { return tapw_.p; }
+    int radius() const { return R_; }
+    float fourier_norm() const { return norm_; }
 
 private:
     Kind kind_;

@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 #include "reduce.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace aspher {
@@ -28,15 +29,53 @@ __device__ __forceinline__ void put(double v, double* red, double* part, int nv,
     (void)nv;
 }
 
+// Ensemble prologue: the element range of this block (a work item of one
+// instance, or the grid-stride range of a single solve), the instance's
+// vector offset, and the gate on its activity flag.
+#define ENS_SETUP(N)                                                                   \
+    std::int64_t i0_, i1_, is_;                                                        \
+    int inst_, row_;                                                                   \
+    if (ens.work) {                                                                    \
+        const WorkItem w_ = ens.work[blockIdx.x];                                      \
+        i0_ = w_.begin + threadIdx.x;                                                  \
+        i1_ = w_.end;                                                                  \
+        is_ = blockDim.x;                                                              \
+        inst_ = w_.inst;                                                               \
+        row_ = w_.row;                                                                 \
+    } else {                                                                           \
+        i0_ = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x;                     \
+        i1_ = (N);                                                                     \
+        is_ = (std::int64_t)gridDim.x * blockDim.x;                                    \
+        inst_ = 0;                                                                     \
+        row_ = blockIdx.x;                                                             \
+    }                                                                                  \
+    if (ens.gate >= 0 && ens.gscal[(std::size_t)inst_ * ens.sstride + ens.gate] == 0.0) \
+        return;                                                                        \
+    const std::int64_t o_ = ens.off ? ens.off[inst_] : 0;                             \
+    (void)o_;                                                                          \
+    (void)row_
+#define ENS_LOOP(i) for (std::int64_t i = i0_; i < i1_; i += is_)
+#define ENS_SCAL(p) ((p) + (std::size_t)inst_ * ens.sstride)
+#define ENS_PART(p) ((p) + (std::size_t)inst_ * ens.pstride + (std::size_t)row_ * kPartVals)
+
+template <int OP>
+__device__ __forceinline__ void putr(double v, double* red, double* prow, int idx) {
+    v = block_reduce<OP>(v, red);
+    if (threadIdx.x == 0) prow[idx] = v;
+}
+
 // ── evaluate (src/contact_solver.cpp:260-303) ──────────────────────────────
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_eval1(std::int64_t N, const T* __restrict__ u, const T* __restrict__ g0s,
-        const T* __restrict__ p, T* __restrict__ g, double* __restrict__ part) {
+        const T* __restrict__ p, T* __restrict__ g, double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
+    ENS_SETUP(N);
+    u += o_; g0s += o_; p += o_; g += o_;
+    double* pr = ENS_PART(part);
     double gsum = 0.0, nc = 0.0, vmin = __builtin_huge_val();
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         const T v = u[i] + g0s[i];
         g[i] = v;
         const double vd = (double)v;
@@ -46,21 +85,25 @@ k_eval1(std::int64_t N, const T* __restrict__ u, const T* __restrict__ g0s,
             nc += 1.0;
         }
     }
-    put<RSUM>(gsum, red, part, 3, 0);
-    put<RSUM>(nc, red, part, 3, 1);
-    put<RMIN>(vmin, red, part, 3, 2);
+    putr<RSUM>(gsum, red, pr, 0);
+    putr<RSUM>(nc, red, pr, 1);
+    putr<RMIN>(vmin, red, pr, 2);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
         std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
-        double* __restrict__ part) {
+        double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
+    ENS_SETUP(N);
+    p += o_; g += o_; mask += o_;
+    scal = ENS_SCAL(scal);
+    double* pr = ENS_PART(part);
     const T alpha = static_cast<T>(scal[S_ALPHA]);
     const T gmin = static_cast<T>(scal[S_GMIN]);
     double pv = 0.0, gcs = 0.0, e = 0.0, Gc = 0.0, psum = 0.0;
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         const T pi = p[i];
         const double pd = (double)pi;
         T gi = g[i];
@@ -75,18 +118,19 @@ k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
         psum += pd;
         mask[i] = c ? 1 : 0;
     }
-    put<RSUM>(pv, red, part, 5, 0);
-    put<RSUM>(gcs, red, part, 5, 1);
-    put<RSUM>(e, red, part, 5, 2);
-    put<RSUM>(Gc, red, part, 5, 3);
-    put<RSUM>(psum, red, part, 5, 4);
+    putr<RSUM>(pv, red, pr, 0);
+    putr<RSUM>(gcs, red, pr, 1);
+    putr<RSUM>(e, red, pr, 2);
+    putr<RSUM>(Gc, red, pr, 3);
+    putr<RSUM>(psum, red, pr, 4);
 }
 
 // ── stencil preconditioner apply_full (src/stencil_precond.cpp:63-109) ────
-// Tiles of 64 x 16 outputs (grid-stride over tiles): the centred, masked
-// residual of the tile plus an R-cell periodic halo is staged once in LDS
-// (FP64), then every contact point sums its taps from LDS in the
-// reference's tap order.
+// Tiles of 64 x 16 outputs: the centred, masked residual of the tile plus an
+// R-cell periodic halo is staged once in LDS (FP64), then every contact
+// point sums its taps from LDS in the reference's tap order. A single solve
+// walks the tiles grid-stride; an ensemble takes one tile per work item
+// (begin = tile index, row = partial row).
 constexpr int kStX = 64, kStY = 16;
 
 template <typename T>
@@ -94,16 +138,35 @@ __global__ void __launch_bounds__(256)
 k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask,
           const double* __restrict__ scal, const int* __restrict__ tapd,
           const double* __restrict__ tapw, int ntaps, int R, T* __restrict__ z,
-          double* __restrict__ part) {
+          double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
     extern __shared__ __align__(16) unsigned char smem[];
     double* tile = reinterpret_cast<double*>(smem);
+    int t0, t1, ts, inst = 0, row = blockIdx.x;
+    if (ens.work) {
+        const WorkItem w = ens.work[blockIdx.x];
+        t0 = (int)w.begin;
+        t1 = (int)w.end;
+        ts = 1;
+        inst = w.inst;
+        row = w.row;
+        Ns = ens.ns[inst];
+    } else {
+        t0 = blockIdx.x;
+        t1 = ((Ns + kStX - 1) / kStX) * ((Ns + kStY - 1) / kStY);
+        ts = gridDim.x;
+    }
+    if (ens.gate >= 0 && ens.gscal[(std::size_t)inst * ens.sstride + ens.gate] == 0.0) return;
+    const std::int64_t o = ens.off ? ens.off[inst] : 0;
+    g += o; mask += o; z += o;
+    scal += (std::size_t)inst * ens.sstride;
+    double* pr = part + (std::size_t)inst * ens.pstride + (std::size_t)row * kPartVals;
     const int HX = kStX + 2 * R, HY = kStY + 2 * R;
     const double gmean = scal[S_GMEAN];
-    const int ntx = (Ns + kStX - 1) / kStX, nty = (Ns + kStY - 1) / kStY;
+    const int ntx = (Ns + kStX - 1) / kStX;
     const int tx = threadIdx.x & 63, ty0 = threadIdx.x >> 6;
     double zsum = 0.0;
-    for (int t = blockIdx.x; t < ntx * nty; t += gridDim.x) {
+    for (int t = t0; t < t1; t += ts) {
         const int x0 = (t % ntx) * kStX, y0 = (t / ntx) * kStY;
         __syncthreads();
         for (int e = threadIdx.x; e < HX * HY; e += blockDim.x) {
@@ -132,7 +195,7 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
             zsum += acc;
         }
     }
-    put<RSUM>(zsum, red, part, 1, 0);
+    putr<RSUM>(zsum, red, pr, 0);
 }
 
 // ── CG direction (src/contact_solver.cpp:411-440) ──────────────────────────
@@ -141,11 +204,16 @@ template <typename T, bool PC>
 __global__ void __launch_bounds__(256)
 k_cgdot(std::int64_t N, const T* __restrict__ g, const T* __restrict__ gprev,
         const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
-        const double* __restrict__ scal, double* __restrict__ part) {
+        const double* __restrict__ scal, double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
+    ENS_SETUP(N);
+    g += o_; gprev += o_; mask += o_;
+    if (PC) z += o_;
+    scal = ENS_SCAL(scal);
+    double* pr = ENS_PART(part);
     const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
     double G = 0.0, Gpr = 0.0, gg = 0.0;
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         if (!mask[i]) continue;
         const T gi = g[i];
         const T zi = PC ? T(z[i] - zmean) : gi;
@@ -154,19 +222,23 @@ k_cgdot(std::int64_t N, const T* __restrict__ g, const T* __restrict__ gprev,
         Gpr += zd * (double)(T)(gi - gprev[i]);
         gg += gd * gd;
     }
-    put<RSUM>(G, red, part, 3, 0);
-    put<RSUM>(Gpr, red, part, 3, 1);
-    put<RSUM>(gg, red, part, 3, 2);
+    putr<RSUM>(G, red, pr, 0);
+    putr<RSUM>(Gpr, red, pr, 1);
+    putr<RSUM>(gg, red, pr, 2);
 }
 
 template <typename T, bool PC>
 __global__ void __launch_bounds__(256)
 k_tupdate(std::int64_t N, const T* __restrict__ g, T* __restrict__ gprev,
           const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
-          T* __restrict__ t, const double* __restrict__ scal) {
+          T* __restrict__ t, const double* __restrict__ scal, Ens ens) {
+    ENS_SETUP(N);
+    g += o_; gprev += o_; mask += o_; t += o_;
+    if (PC) z += o_;
+    scal = ENS_SCAL(scal);
     const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
     const T beta = static_cast<T>(scal[S_BETA]);
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         const T gi = g[i];
         if (mask[i]) {
             const T zi = PC ? T(z[i] - zmean) : gi;
@@ -184,31 +256,38 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_line1(std::int64_t N, const T* __restrict__ r, const T* __restrict__ g,
         const T* __restrict__ t, const std::uint8_t* __restrict__ mask,
-        double* __restrict__ part) {
+        double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
+    ENS_SETUP(N);
+    r += o_; g += o_; t += o_; mask += o_;
+    double* pr = ENS_PART(part);
     double rsum = 0.0, num = 0.0;
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         if (!mask[i]) continue;
         rsum += (double)r[i];
         num += (double)g[i] * (double)t[i];
     }
-    put<RSUM>(rsum, red, part, 2, 0);
-    put<RSUM>(num, red, part, 2, 1);
+    putr<RSUM>(rsum, red, pr, 0);
+    putr<RSUM>(num, red, pr, 1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_line2(std::int64_t N, const T* __restrict__ r, const T* __restrict__ t,
         const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
-        double* __restrict__ part) {
+        double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
+    ENS_SETUP(N);
+    r += o_; t += o_; mask += o_;
+    scal = ENS_SCAL(scal);
+    double* pr = ENS_PART(part);
     const T rmean = static_cast<T>(scal[S_RMEAN]);
     double den = 0.0;
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         if (!mask[i]) continue;
         den += (double)(T)(r[i] - rmean) * (double)t[i];
     }
-    put<RSUM>(den, red, part, 1, 0);
+    putr<RSUM>(den, red, pr, 0);
 }
 
 // ── update, overlap step, load rescale (src/contact_solver.cpp:477-501) ────
@@ -217,19 +296,23 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_update(std::int64_t N, T* __restrict__ p, const T* __restrict__ t,
          const T* __restrict__ g, const double* __restrict__ scal,
-         double* __restrict__ part) {
+         double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
+    ENS_SETUP(N);
+    p += o_; t += o_; g += o_;
+    scal = ENS_SCAL(scal);
+    double* pr = ENS_PART(part);
     if (scal[S_OK] == 0.0) {
         if (threadIdx.x == 0) {
-            part[(std::size_t)blockIdx.x * kPartVals + 0] = 0.0;
-            part[(std::size_t)blockIdx.x * kPartVals + 1] = 0.0;
+            pr[0] = 0.0;
+            pr[1] = 0.0;
         }
         return;
     }
     const T tau = static_cast<T>(scal[S_TAU]);
     const T tau_ov = static_cast<T>(scal[S_TAUOV]);
     double changed = 0.0, total = 0.0;
-    GRID_LOOP(i, N) {
+    ENS_LOOP(i) {
         T pi = p[i];
         const bool was = pi > T(0);
         if (was) {
@@ -242,18 +325,33 @@ k_update(std::int64_t N, T* __restrict__ p, const T* __restrict__ t,
         p[i] = pi;
         total += (double)pi;
     }
-    put<RMAX>(changed, red, part, 2, 0);
-    put<RSUM>(total, red, part, 2, 1);
+    putr<RMAX>(changed, red, pr, 0);
+    putr<RSUM>(total, red, pr, 1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_rescale(std::int64_t N, T* __restrict__ p, const double* __restrict__ scal) {
+k_rescale(std::int64_t N, T* __restrict__ p, const double* __restrict__ scal, Ens ens) {
+    ENS_SETUP(N);
+    p += o_;
+    scal = ENS_SCAL(scal);
     if (scal[S_OK] == 0.0) return;
     const bool fill = scal[S_PFILL] != 0.0;
     const T f = static_cast<T>(scal[S_F]);
     const T pbar = static_cast<T>(scal[S_PBAR]);
-    GRID_LOOP(i, N) p[i] = fill ? pbar : T(p[i] * f);
+    ENS_LOOP(i) p[i] = fill ? pbar : T(p[i] * f);
+}
+
+// best iterate: y = x where the instance's scalar `flag` is nonzero
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_copy_if(std::int64_t N, const T* __restrict__ x, T* __restrict__ y,
+          const double* __restrict__ scal, int flag, Ens ens) {
+    ENS_SETUP(N);
+    x += o_; y += o_;
+    scal = ENS_SCAL(scal);
+    if (scal[flag] == 0.0) return;
+    ENS_LOOP(i) y[i] = x[i];
 }
 
 // ── identification step and simplex projection (src/contact_solver.cpp:31-60,314-364)
@@ -432,6 +530,63 @@ k_dense(int Ns, const double* __restrict__ table, const T* __restrict__ x, T* __
 
 // ── finalize: partials -> scalars, then the derived-scalar step ────────────
 
+
+// Ensemble iteration control after a fresh evaluation: the loop body of
+// src/contact_solver.cpp:366-407 (convergence, best iterate, stall) decided
+// on the device, so an instance leaves the replayed iteration by itself.
+// Branches that need the host (identification step after a failed line
+// search or with fewer than two contact points, stall, iteration budget)
+// clear S_ACTIVE and record the reason in S_HOST.
+__device__ void ens_control(double* s) {
+    const bool pend = s[S_PENDING] != 0.0;
+    s[S_PENDING] = 0.0;
+    s[S_SAVEBEST] = 0.0;
+    if (pend && s[S_OK] == 0.0) {
+        s[S_ACTIVE] = 0.0;
+        s[S_HOST] = HOST_IDENT;
+        return;
+    }
+    if (s[S_FINITE] == 0.0) {
+        s[S_ACTIVE] = 0.0;
+        s[S_HOST] = HOST_NONFINITE;
+        return;
+    }
+    const double fw = s[S_FW], pen = s[S_PEN];
+    if (s[S_LOAD] <= s[S_LOADTOL] && fw <= s[S_TOL] && pen <= s[S_TOL]) {
+        s[S_ACTIVE] = 0.0;
+        s[S_STATUS] = 1.0;   // converged
+        return;
+    }
+    const double merit = fw > pen ? fw : pen;
+    if (s[S_KEEPBEST] != 0.0 && merit < s[S_BEST]) {
+        s[S_BEST] = merit;
+        s[S_SAVEBEST] = 1.0;
+    }
+    if (merit < s[S_SEEN] * (1.0 - 1e-4)) {
+        s[S_SEEN] = merit;
+        s[S_STALL] = 0.0;
+    } else {
+        s[S_STALL] += 1.0;
+        if (s[S_STALL] >= s[S_STALL_LIMIT]) {
+            s[S_ACTIVE] = 0.0;
+            s[S_HOST] = HOST_STALL;
+            return;
+        }
+    }
+    if (s[S_ITER] >= s[S_MAXIT]) {
+        s[S_ACTIVE] = 0.0;
+        s[S_HOST] = HOST_MAXIT;
+        return;
+    }
+    if (s[S_NC] < 2.0) {
+        s[S_ACTIVE] = 0.0;
+        s[S_HOST] = HOST_IDENT;
+        return;
+    }
+    s[S_ITER] += 1.0;
+    s[S_PENDING] = 1.0;
+}
+
 template <typename T>
 __device__ void run_step(int step, double* s) {
     const double nc = s[S_NC];
@@ -443,7 +598,8 @@ __device__ void run_step(int step, double* s) {
             s[S_GMIN] = s[S_VMIN] - alpha;
             break;
         }
-        case STEP_EVAL2: {
+        case STEP_EVAL2:
+        case STEP_B_EVAL2: {
             double a = s[S_ALPHA], gm = s[S_GMIN];
             if (nc > 0.0) {  // refine_contact_mean (src/contact_solver.cpp:94-99)
                 const double delta = s[S_GCS] / nc;
@@ -460,6 +616,7 @@ __device__ void run_step(int step, double* s) {
                            isfinite(s[S_PSUM]) && isfinite(s[S_E]) && isfinite(s[S_GC]))
                               ? 1.0 : 0.0;
             s[S_GMEAN] = nc > 0.0 ? s[S_GCS] / nc : 0.0;
+            if (step == STEP_B_EVAL2) ens_control(s);
             break;
         }
         case STEP_ZMEAN:
@@ -550,8 +707,14 @@ struct FinSpec {
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __restrict__ scal,
-           int step) {
+           int step, Ens ens) {
     __shared__ double red[8];
+    // one block per instance (a single solve: one block)
+    const int inst = blockIdx.x;
+    if (ens.gate >= 0 && ens.gscal[(std::size_t)inst * ens.sstride + ens.gate] == 0.0) return;
+    part += (std::size_t)inst * ens.pstride;
+    scal += (std::size_t)inst * ens.sstride;
+    if (ens.rows) nblk = ens.rows[inst];
     for (int v = 0; v < spec.nv; ++v) {
         const int op = spec.ops[v];
         double acc = op == RSUM ? 0.0 : (op == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
@@ -772,61 +935,84 @@ k_adj_finish(std::int64_t N, const T* __restrict__ x, const T* __restrict__ w,
     put<RSUM>(s, red, part, 1, 0);
 }
 
+
+// Plain streaming copy in 16-byte words (calibration of the HBM counters:
+// reads and writes exactly `n16` * 16 bytes).
+__global__ void __launch_bounds__(256)
+k_copy16(std::int64_t n16, const float4* __restrict__ x, float4* __restrict__ y) {
+    GRID_LOOP(i, n16) y[i] = x[i];
+}
+
 } // namespace
 
 #define L1(k, ...) k<<<nblk, kVecThreads, 0, s>>>(__VA_ARGS__); ASPHER_LAUNCH_CHECK()
 #define L1S(k, ...) k<<<vec_blocks(N), kVecThreads, 0, s>>>(__VA_ARGS__); ASPHER_LAUNCH_CHECK()
+// ensemble-aware launches: one block per work item, else the single-solve grid
+#define LE(k, ...) k<<<ens.work ? ens.nwork : nblk, kVecThreads, 0, s>>>(__VA_ARGS__); \
+    ASPHER_LAUNCH_CHECK()
+#define LES(k, ...) k<<<ens.work ? ens.nwork : vec_blocks(N), kVecThreads, 0, s>>>(__VA_ARGS__); \
+    ASPHER_LAUNCH_CHECK()
 
 template <typename T>
 void VecOps<T>::eval1(std::int64_t N, const T* u, const T* g0s, const T* p, T* g,
-                      double* part, int nblk, hipStream_t s) {
-    L1(k_eval1<T>, N, u, g0s, p, g, part);
+                      double* part, int nblk, hipStream_t s, const Ens& ens) {
+    LE(k_eval1<T>, N, u, g0s, p, g, part, ens);
 }
 template <typename T>
 void VecOps<T>::eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
-                      const double* scal, double* part, int nblk, hipStream_t s) {
-    L1(k_eval2<T>, N, p, g, mask, scal, part);
+                      const double* scal, double* part, int nblk, hipStream_t s,
+                      const Ens& ens) {
+    LE(k_eval2<T>, N, p, g, mask, scal, part, ens);
 }
 template <typename T>
 void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const double* scal,
                         const int* tapd, const double* tapw, int ntaps, int R, T* z,
-                        double* part, int nblk, hipStream_t s) {
+                        double* part, int nblk, hipStream_t s, const Ens& ens) {
     const std::size_t lds = sizeof(double) * (kStX + 2 * R) * (kStY + 2 * R);
-    k_stencil<T><<<nblk, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, ntaps, R, z, part);
+    k_stencil<T><<<ens.work ? ens.nwork : nblk, kVecThreads, lds, s>>>(
+        Ns, g, mask, scal, tapd, tapw, ntaps, R, z, part, ens);
     ASPHER_LAUNCH_CHECK();
 }
 template <typename T>
 void VecOps<T>::cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
                       const std::uint8_t* mask, const double* scal, bool has_pc,
-                      double* part, int nblk, hipStream_t s) {
-    if (has_pc) { L1((k_cgdot<T, true>), N, g, gprev, z, mask, scal, part); }
-    else { L1((k_cgdot<T, false>), N, g, gprev, z, mask, scal, part); }
+                      double* part, int nblk, hipStream_t s, const Ens& ens) {
+    if (has_pc) { LE((k_cgdot<T, true>), N, g, gprev, z, mask, scal, part, ens); }
+    else { LE((k_cgdot<T, false>), N, g, gprev, z, mask, scal, part, ens); }
 }
 template <typename T>
 void VecOps<T>::tupdate(std::int64_t N, const T* g, T* gprev, const T* z,
                         const std::uint8_t* mask, T* t, const double* scal, bool has_pc,
-                        hipStream_t s) {
-    if (has_pc) { L1S((k_tupdate<T, true>), N, g, gprev, z, mask, t, scal); }
-    else { L1S((k_tupdate<T, false>), N, g, gprev, z, mask, t, scal); }
+                        hipStream_t s, const Ens& ens) {
+    if (has_pc) { LES((k_tupdate<T, true>), N, g, gprev, z, mask, t, scal, ens); }
+    else { LES((k_tupdate<T, false>), N, g, gprev, z, mask, t, scal, ens); }
 }
 template <typename T>
 void VecOps<T>::line1(std::int64_t N, const T* r, const T* g, const T* t,
-                      const std::uint8_t* mask, double* part, int nblk, hipStream_t s) {
-    L1(k_line1<T>, N, r, g, t, mask, part);
+                      const std::uint8_t* mask, double* part, int nblk, hipStream_t s,
+                      const Ens& ens) {
+    LE(k_line1<T>, N, r, g, t, mask, part, ens);
 }
 template <typename T>
 void VecOps<T>::line2(std::int64_t N, const T* r, const T* t, const std::uint8_t* mask,
-                      const double* scal, double* part, int nblk, hipStream_t s) {
-    L1(k_line2<T>, N, r, t, mask, scal, part);
+                      const double* scal, double* part, int nblk, hipStream_t s,
+                      const Ens& ens) {
+    LE(k_line2<T>, N, r, t, mask, scal, part, ens);
 }
 template <typename T>
 void VecOps<T>::update(std::int64_t N, T* p, const T* t, const T* g, const double* scal,
-                       double* part, int nblk, hipStream_t s) {
-    L1(k_update<T>, N, p, t, g, scal, part);
+                       double* part, int nblk, hipStream_t s, const Ens& ens) {
+    LE(k_update<T>, N, p, t, g, scal, part, ens);
 }
 template <typename T>
-void VecOps<T>::rescale(std::int64_t N, T* p, const double* scal, hipStream_t s) {
-    L1S(k_rescale<T>, N, p, scal);
+void VecOps<T>::rescale(std::int64_t N, T* p, const double* scal, hipStream_t s,
+                        const Ens& ens) {
+    LES(k_rescale<T>, N, p, scal, ens);
+}
+template <typename T>
+void VecOps<T>::copy_if(std::int64_t N, const T* x, T* y, const double* scal, int flag,
+                        hipStream_t s, const Ens& ens) {
+    LES(k_copy_if<T>, N, x, y, scal, flag, ens);
 }
 template <typename T>
 void VecOps<T>::objective(std::int64_t N, const T* p, const T* u, const T* g0s,
@@ -946,22 +1132,31 @@ void AdjOps<T>::finish(std::int64_t N, const T* x, const T* w, const T* q,
 
 template struct AdjOps<double>;
 
+void copy16(std::int64_t bytes, const void* x, void* y, hipStream_t s) {
+    const std::int64_t n16 = bytes / 16;
+    const int nb = (int)std::min<std::int64_t>((n16 + 255) / 256, 8192);
+    k_copy16<<<std::max(nb, 1), 256, 0, s>>>(n16, static_cast<const float4*>(x),
+                                             static_cast<float4*>(y));
+    ASPHER_LAUNCH_CHECK();
+}
+
 template <typename T>
 void finalize(const double* part, int nblk, int nv, const int* ops, const int* dst,
-              double* scal, Step step, hipStream_t s) {
+              double* scal, Step step, hipStream_t s, const Ens& ens) {
     FinSpec f{};
     f.nv = nv;
     for (int v = 0; v < nv; ++v) {
         f.ops[v] = ops[v];
         f.dst[v] = dst[v];
     }
-    k_finalize<T><<<1, 256, 0, s>>>(part, nblk, f, scal, static_cast<int>(step));
+    k_finalize<T><<<std::max(ens.ninst, 1), 256, 0, s>>>(part, nblk, f, scal,
+                                                         static_cast<int>(step), ens);
     ASPHER_LAUNCH_CHECK();
 }
 template void finalize<float>(const double*, int, int, const int*, const int*, double*,
-                              Step, hipStream_t);
+                              Step, hipStream_t, const Ens&);
 template void finalize<double>(const double*, int, int, const int*, const int*, double*,
-                               Step, hipStream_t);
+                               Step, hipStream_t, const Ens&);
 
 void set_scal(double* scal, int n, const int* idx, const double* val, hipStream_t s) {
     SetSpec sp{};

@@ -25,44 +25,64 @@ enum Scal : int {
     // adjoint CG on a fixed contact set
     S_A_WSUM, S_A_RZ, S_A_RZOLD, S_A_RR, S_A_BETA, S_A_QSUM, S_A_QT, S_A_TSUM,
     S_A_ALPHA, S_A_QMEAN, S_A_OK, S_A_RSUM, S_A_WQSUM, S_A_NU,
+    // device-side iteration control of ensemble solves
+    S_ACTIVE, S_HOST, S_STATUS, S_PENDING, S_ITER, S_MAXIT, S_TOL, S_LOADTOL,
+    S_BEST, S_SAVEBEST, S_SEEN, S_STALL, S_STALL_LIMIT, S_KEEPBEST,
     S_COUNT
+};
+
+// S_HOST codes: why an ensemble instance left the device loop unconverged
+enum HostReason : int {
+    HOST_NONE = 0, HOST_IDENT = 1, HOST_STALL = 2, HOST_MAXIT = 3, HOST_NONFINITE = 4,
 };
 
 enum Step : int {
     STEP_NONE = 0, STEP_EVAL1, STEP_EVAL2, STEP_ZMEAN, STEP_CGDOT, STEP_LINE1,
     STEP_LINE2, STEP_UPDATE,
     STEP_ADJ_INIT, STEP_ADJ_DOT, STEP_ADJ_CURV, STEP_ADJ_RMEAN, STEP_ADJ_FINISH,
+    // STEP_EVAL2 followed by the ensemble's convergence / stall / best-iterate
+    // control, which clears S_ACTIVE when the instance leaves the loop
+    STEP_B_EVAL2,
 };
 
 template <typename T> struct VecOps {
     // evaluate pass 1: g = u + g0s; partials {sum_{p>0} v, n_{p>0}, min v}
     static void eval1(std::int64_t N, const T* u, const T* g0s, const T* p, T* g,
-                      double* part, int nblk, hipStream_t s);
+                      double* part, int nblk, hipStream_t s, const Ens& ens = Ens{});
     // evaluate pass 2 (uses S_ALPHA, S_GMIN): partials {pv, gcs, e, Gc, psum};
     // writes mask = p > 0
     static void eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
-                      const double* scal, double* part, int nblk, hipStream_t s);
+                      const double* scal, double* part, int nblk, hipStream_t s,
+                      const Ens& ens = Ens{});
     // 13-tap (any radius) masked stencil (uses S_GMEAN): z and partial zsum
     static void stencil(int Ns, const T* g, const std::uint8_t* mask,
                         const double* scal, const int* tapd, const double* tapw,
-                        int ntaps, int R, T* z, double* part, int nblk, hipStream_t s);
+                        int ntaps, int R, T* z, double* part, int nblk, hipStream_t s,
+                        const Ens& ens = Ens{});
     // {G, G_pr, gg} over the mask; z is (z - zmean) when has_pc, else g
     static void cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
                       const std::uint8_t* mask, const double* scal, bool has_pc,
-                      double* part, int nblk, hipStream_t s);
+                      double* part, int nblk, hipStream_t s, const Ens& ens = Ens{});
     // t = mask ? z + beta t : 0 ; gprev = g
     static void tupdate(std::int64_t N, const T* g, T* gprev, const T* z,
                         const std::uint8_t* mask, T* t, const double* scal,
-                        bool has_pc, hipStream_t s);
+                        bool has_pc, hipStream_t s, const Ens& ens = Ens{});
     static void line1(std::int64_t N, const T* r, const T* g, const T* t,
-                      const std::uint8_t* mask, double* part, int nblk, hipStream_t s);
+                      const std::uint8_t* mask, double* part, int nblk, hipStream_t s,
+                      const Ens& ens = Ens{});
     static void line2(std::int64_t N, const T* r, const T* t, const std::uint8_t* mask,
-                      const double* scal, double* part, int nblk, hipStream_t s);
+                      const double* scal, double* part, int nblk, hipStream_t s,
+                      const Ens& ens = Ens{});
     // projected update + overlap step (predicated on S_OK): partials {changed, total}
     static void update(std::int64_t N, T* p, const T* t, const T* g,
-                       const double* scal, double* part, int nblk, hipStream_t s);
+                       const double* scal, double* part, int nblk, hipStream_t s,
+                       const Ens& ens = Ens{});
     // load rescale (predicated on S_OK)
-    static void rescale(std::int64_t N, T* p, const double* scal, hipStream_t s);
+    static void rescale(std::int64_t N, T* p, const double* scal, hipStream_t s,
+                        const Ens& ens = Ens{});
+    // y = x where scal[flag] != 0 (per instance)
+    static void copy_if(std::int64_t N, const T* x, T* y, const double* scal, int flag,
+                        hipStream_t s, const Ens& ens = Ens{});
     // {sum p(0.5u+g0s), sum p^2, sum p u}
     static void objective(std::int64_t N, const T* p, const T* u, const T* g0s,
                           double* part, int nblk, hipStream_t s);
@@ -129,7 +149,7 @@ template <typename T> struct AdjOps {
 // a finalize step. T selects the working-precision casts of the step.
 template <typename T>
 void finalize(const double* part, int nblk, int nv, const int* ops, const int* dst,
-              double* scal, Step step, hipStream_t s);
+              double* scal, Step step, hipStream_t s, const Ens& ens = Ens{});
 
 // Set a few scalars from the host (a kernel, so the stream order holds).
 void set_scal(double* scal, int n, const int* idx, const double* val, hipStream_t s);
@@ -143,6 +163,9 @@ void love_points(int n, const double* x, const double* y, const double* a,
 // 2x2 block-average restriction (Ns -> Ns/2) and injection prolongation.
 void restrict2(int Ns, const double* f, double* out, hipStream_t s);
 void prolong2(int Nc, const double* pc, double* out, hipStream_t s);
+
+// plain device copy of `bytes` (a multiple of 16) with a kernel of this library
+void copy16(std::int64_t bytes, const void* x, void* y, hipStream_t s);
 
 // float <-> double spectrum copy
 void cast_d2f(std::int64_t N, const double* x, float* y, hipStream_t s);

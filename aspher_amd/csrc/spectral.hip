@@ -348,9 +348,10 @@ Twiddles<T>::~Twiddles() {
 
 template <typename T>
 SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
-                              SpectralTuning tune)
-    : n(n_), rows_in(rin), cols_in(cin), rows_out(rout), cols_out(cout) {
+                              SpectralTuning tune, int batch)
+    : n(n_), rows_in(rin), cols_in(cin), rows_out(rout), cols_out(cout), batch_(batch) {
     if (n < 2) throw std::invalid_argument("transform size must be >= 2");
+    if (batch < 1) throw std::invalid_argument("batch must be >= 1");
     nh = n / 2 + 1;
     pitch = (nh + 15) / 16 * 16;
     const bool packed = (n % 2 == 0);
@@ -378,10 +379,14 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
         pitchT_ = (std::max(rows_in, 1) + 31) / 32 * 32 + 32;
         tw64_ = std::make_unique<Twiddles<double>>(n);
         a_rows_ = static_cast<std::size_t>(std::max(rows_out, 1));
-        ASPHER_HIP(hipMalloc(&A_, cs * a_rows_ * pitch));
-        ASPHER_HIP(hipMalloc(&AT_, cs * (std::size_t)nh * pitchT_));
+        br_stride_ = (std::int64_t)a_rows_ * pitch;
+        at_stride_ = (std::int64_t)nh * pitchT_;
+        ASPHER_HIP(hipMalloc(&A_, cs * (std::size_t)br_stride_ * batch_));
+        ASPHER_HIP(hipMalloc(&AT_, cs * (std::size_t)at_stride_ * batch_));
         return;
     }
+    if (batch_ != 1)
+        throw std::invalid_argument("batched transforms need a power-of-two length in [32, 8192]");
     if (pow2_) {
         row_batch_ = tune.row_batch > 0 ? tune.row_batch : std::max(1, 2048 / Lr);
         while (Lr * row_batch_ > 8192) row_batch_ /= 2;
@@ -415,8 +420,8 @@ SpectralPlan<T>::~SpectralPlan() {
 
 template <typename T>
 std::size_t SpectralPlan<T>::workspace_bytes() const {
-    return sizeof(cx<T>) * a_rows_ * pitch +
-           (rb_ ? sizeof(cx<double>) * n + sizeof(cx<T>) * (std::size_t)nh * pitchT_
+    return sizeof(cx<T>) * a_rows_ * pitch * batch_ +
+           (rb_ ? sizeof(cx<double>) * n + sizeof(cx<T>) * (std::size_t)nh * pitchT_ * batch_
                 : sizeof(cx<T>) * n);
 }
 
@@ -535,9 +540,25 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
                                const T* ktab, int ktab_pitch, double sym_scale,
                                hipStream_t s, RowIn in_mode, RowOut out_mode,
                                const std::uint8_t* mask, const double* scal,
-                               int gmean_idx, double* part, hipEvent_t* ev) {
+                               int gmean_idx, double* part, hipEvent_t* ev,
+                               const SpecBatch* sb) {
     if (sym == Symbol::build) throw std::invalid_argument("convolve: build symbol");
+    if (sb && (!rb_ || sb->count > batch_ || sb->count < 1))
+        throw std::invalid_argument("convolve: batch exceeds the plan");
     if (rb_) {
+        RbBatch bt;
+        if (sb) {
+            bt.count = sb->count;
+            bt.x_stride = sb->x_stride;
+            bt.y_stride = sb->y_stride;
+            bt.at_stride = at_stride_;
+            bt.br_stride = br_stride_;
+            bt.first = sb->first;
+            bt.sstride = sb->sstride;
+            bt.pstride = sb->pstride;
+            bt.gscal = sb->gscal;
+            bt.gate = sb->gate;
+        }
         const int Lr = n / 2;
         RbRowsFwd<T> f;
         f.x = x;
@@ -553,7 +574,9 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         f.scal = scal;
         f.gmean_idx = gmean_idx;
         f.remap = xcd_remap_ ? 1 : 0;
+        f.bt = bt;
         RbCols<T> c;
+        c.bt = bt;
         c.AT = AT_;
         c.pitchT = pitchT_;
         c.BR = A_;
@@ -581,6 +604,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         r.mask = mask;
         r.part = part;
         r.remap = f.remap;
+        r.bt = bt;
         if (ev) ASPHER_HIP(hipEventRecord(ev[0], s));
         rb_rows_fwd<T>(ilog2(Lr), f, s);
         if (ev) ASPHER_HIP(hipEventRecord(ev[1], s));

@@ -45,6 +45,19 @@ struct SpectralTuning {
     bool xcd_remap = true;
 };
 
+// Equal-size instances of one convolve launch (ensemble groups, register-
+// blocked sizes only): instance l reads x + l*x_stride, writes y + l*y_stride
+// and uses the scalar/partial blocks of global instance first + l; instances
+// whose gate flag is 0 are skipped.
+struct SpecBatch {
+    int count = 1;
+    std::int64_t x_stride = 0, y_stride = 0;
+    int first = 0, sstride = 0;
+    std::int64_t pstride = 0;
+    const double* gscal = nullptr;
+    int gate = -1;
+};
+
 // Twiddle table exp(-2 pi i t/n), t in [0, n), built on the host in long
 // double and uploaded once per (n, precision).
 template <typename T>
@@ -63,7 +76,7 @@ template <typename T>
 class SpectralPlan {
 public:
     SpectralPlan(int n, int rows_in, int cols_in, int rows_out, int cols_out,
-                 SpectralTuning tune = {});
+                 SpectralTuning tune = {}, int batch = 1);
     ~SpectralPlan();
     SpectralPlan(const SpectralPlan&) = delete;
     SpectralPlan& operator=(const SpectralPlan&) = delete;
@@ -78,7 +91,8 @@ public:
                   RowOut out_mode = RowOut::plain,
                   const std::uint8_t* mask = nullptr,
                   const double* scal = nullptr, int gmean_idx = 0,
-                  double* part = nullptr, hipEvent_t* stage_events = nullptr);
+                  double* part = nullptr, hipEvent_t* stage_events = nullptr,
+                  const SpecBatch* batch = nullptr);
 
     // Forward-only pass used to build the free-space kernel spectrum from
     // the Love table (T = double): writes Re/n^2 into kout[(kx)(n/2+1)+ky]
@@ -87,6 +101,8 @@ public:
                         hipStream_t s);
 
     int row_out_blocks() const;   // blocks of rows_inv (partial count)
+    int batch_capacity() const { return batch_; }
+    SpectralTuning tuning() const { return {row_batch_, col_width_, xcd_remap_}; }
     int col_blocks() const;       // blocks of cols (partial count, build)
     std::size_t workspace_bytes() const;
 
@@ -112,6 +128,8 @@ private:
     int pitchT_ = 0;
     std::unique_ptr<Twiddles<double>> tw64_;
     int logL_row_ = 0, logn_ = 0;
+    int batch_ = 1;
+    std::int64_t at_stride_ = 0, br_stride_ = 0;
     int row_batch_ = 1, col_width_ = 1;
     bool xcd_remap_ = true;
     std::vector<int> rad_row_, rad_col_;

@@ -34,11 +34,20 @@ __global__ void __launch_bounds__(512)
 k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
               cx<T>* __restrict__ AT, int pitchT, int B, const cx<double>* __restrict__ tw,
               const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
-              int gmean_idx, const double* __restrict__ table, int Ns_embed, int remap) {
+              int gmean_idx, const double* __restrict__ table, int Ns_embed, int remap,
+              RbBatch bt) {
     using Sh = rb::Shape<LOGL>;
     constexpr int L = Sh::n, n = 2 * L, TPR = Sh::threads;
     constexpr int LL = rb::lds_len(L);
     extern __shared__ __align__(16) unsigned char smem[];
+    {
+        const int li = blockIdx.y, gi = bt.first + li;
+        if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
+        x += li * bt.x_stride;
+        if (mask) mask += li * bt.x_stride;
+        AT += li * bt.at_stride;
+        if (scal) scal += (std::size_t)gi * bt.sstride;
+    }
     cx<T>* lds_all = reinterpret_cast<cx<T>*>(smem);
     const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int b = threadIdx.x / TPR, th = threadIdx.x - b * TPR;
@@ -123,8 +132,14 @@ __global__ void __launch_bounds__(512)
 k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int pitch,
           int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
           const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
-          double* __restrict__ part, int remap) {
+          double* __restrict__ part, int remap, RbBatch bt) {
     using Sh = rb::Shape<LOGN>;
+    {
+        const int li = blockIdx.y, gi = bt.first + li;
+        if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
+        AT += li * bt.at_stride;
+        BR += li * bt.br_stride;
+    }
     constexpr int n = Sh::n, TPC = Sh::threads;
     constexpr int LL = rb::lds_len(n);
     constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
@@ -192,8 +207,17 @@ template <typename T, int LOGL, int OUT>
 __global__ void __launch_bounds__(512)
 k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
               const cx<double>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
-              const std::uint8_t* __restrict__ mask, double* __restrict__ part, int remap) {
+              const std::uint8_t* __restrict__ mask, double* __restrict__ part, int remap,
+              RbBatch bt) {
     using Sh = rb::Shape<LOGL>;
+    {
+        const int li = blockIdx.y, gi = bt.first + li;
+        if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
+        BR += li * bt.br_stride;
+        y += li * bt.y_stride;
+        if (mask) mask += li * bt.y_stride;
+        if (part) part += (std::size_t)gi * bt.pstride;
+    }
     constexpr int L = Sh::n, TPR = Sh::threads;
     constexpr int LL = rb::lds_len(L);
     constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
@@ -273,11 +297,11 @@ template <typename T, int LOGL>
 void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
     constexpr int L = 1 << LOGL;
     const int block = a.B * (L / 16);
-    const int grid = (a.nrows + a.B - 1) / a.B;
+    const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
     const void* fn;
 #define RF_ARGS a.x, a.x_pitch, a.ncols, a.nrows, a.AT, a.pitchT, a.B, a.tw, a.mask, a.scal, \
-        a.gmean_idx, a.table, a.Ns_embed, a.remap
+        a.gmean_idx, a.table, a.Ns_embed, a.remap, a.bt
     if (a.mode == 0) {
         fn = (const void*)k_rb_rows_fwd<T, LOGL, 0>;
         allow_lds_rb(fn, lds);
@@ -299,10 +323,10 @@ template <typename T, int LOGN>
 void launch_cols(const RbCols<T>& a, hipStream_t s) {
     constexpr int n = 1 << LOGN;
     const int block = a.W * (n / 16);
-    const int grid = (a.ncols + a.W - 1) / a.W;
+    const dim3 grid((a.ncols + a.W - 1) / a.W, a.bt.count);
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.W * rb::lds_len(n);
 #define RC_ARGS a.AT, a.pitchT, a.BR, a.pitch, a.ncols, a.rows_in, a.rows_out, a.W, a.tw, a.ktab, \
-        a.kp, a.sc, a.kout, a.part, a.remap
+        a.kp, a.sc, a.kout, a.part, a.remap, a.bt
 #define RC_CASE(S)                                                     \
     case S: {                                                          \
         const void* fn = (const void*)k_rb_cols<T, LOGN, S>;           \
@@ -326,9 +350,10 @@ template <typename T, int LOGL>
 void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
     constexpr int L = 1 << LOGL;
     const int block = a.B * (L / 16);
-    const int grid = (a.nrows + a.B - 1) / a.B;
+    const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
-#define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap
+#define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap, \
+        a.bt
     if (a.mode == 0) {
         const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0>;
         allow_lds_rb(fn, lds);

@@ -7,6 +7,20 @@
 
 namespace aspher {
 
+// Instances of one launch (blockIdx.y): equal-size problems of an ensemble
+// group. Local instance l uses x + l*x_stride (and its mask), y + l*y_stride,
+// its own slice of the plan workspace, and the scalar/partial blocks of
+// global instance first + l; instances whose gate flag is 0 are skipped.
+struct RbBatch {
+    int count = 1;
+    std::int64_t x_stride = 0, y_stride = 0, at_stride = 0, br_stride = 0;
+    int first = 0;
+    int sstride = 0;
+    std::int64_t pstride = 0;
+    const double* gscal = nullptr;
+    int gate = -1;
+};
+
 template <typename T>
 struct RbRowsFwd {
     const T* x = nullptr;
@@ -22,6 +36,7 @@ struct RbRowsFwd {
     const double* table = nullptr;
     int Ns_embed = 0;
     int remap = 1;
+    RbBatch bt;
 };
 
 template <typename T>
@@ -40,6 +55,7 @@ struct RbCols {
     T* kout = nullptr;
     double* part = nullptr;
     int remap = 1;
+    RbBatch bt;
 };
 
 template <typename T>
@@ -54,6 +70,7 @@ struct RbRowsInv {
     const std::uint8_t* mask = nullptr;
     double* part = nullptr;
     int remap = 1;
+    RbBatch bt;
 };
 
 template <typename T> void rb_rows_fwd(int logL, const RbRowsFwd<T>& a, hipStream_t s);

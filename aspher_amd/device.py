@@ -38,6 +38,12 @@ class DeviceArray:
         check(lib().aspher_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self._p, self.nbytes))
         return out
 
+    def copy_from(self, other: "DeviceArray") -> None:
+        """Device-to-device copy with the library's streaming copy kernel."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(lib().aspher_copy_device(self._p, other._p, self.nbytes))
+
     @property
     def ptr(self) -> int:
         return self._p.value

@@ -101,6 +101,50 @@ def cpu_baseline(Ns, gap, fp64_stages, workers):
                 sample_s=t_build + 2 * t_mv + t_solve)
 
 
+def ensemble_problems(hc, n=64, seed=0):
+    """The batched workload: sizes from {256, 512, 1024, 2048}^2, Hurst
+    uniform in [0.6, 0.9], rms 0.002, |k| in [4, 64], p_nominal log-uniform in
+    [0.001, 0.02], free-space / periodic 50/50, all drawn from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for k in range(n):
+        Ns = int(rng.choice([256, 512, 1024, 2048]))
+        H = float(rng.uniform(0.6, 0.9))
+        pbar = float(np.exp(rng.uniform(np.log(0.001), np.log(0.02))))
+        backend = "fft" if rng.random() < 0.5 else "periodic"
+        probs.append(dict(backend=backend, grid_size=Ns, p_nominal=pbar,
+                          gap=hc.surfaces.rough_gap(Ns, H=H, k_low=4, k_high=64, rms=0.002,
+                                                    seed=1000 + k)))
+    return probs
+
+
+def run_ensemble(hc, DeviceArray, steps, tol=1e-5):
+    probs = ensemble_problems(hc)
+    dev = [DeviceArray.from_numpy(p["gap"], np.float64) for p in probs]
+    dprobs = [dict(p, gap_ptr=d.ptr) for p, d in zip(probs, dev)]
+    for p in dprobs:
+        p.pop("gap")
+    hc.solve_ensemble_device(dprobs, tol=tol)             # warm-up: builds + autotuning
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res, st = hc.solve_ensemble_device(dprobs, tol=tol)
+    dt = (time.perf_counter() - t0) / steps
+    conv = sum(r.status == "converged" for r in res)
+    lat = {}
+    for p, t, r in zip(probs, st["instance_seconds"], res):
+        lat.setdefault("%s %d" % (p["backend"], p["grid_size"]), []).append((t, r.iterations))
+    return dict(solves_per_s=round(len(probs) / dt, 3), seconds=round(dt, 4),
+                instances=len(probs), converged=conv, tol=tol, precision="float",
+                loop_s=round(st["loop_seconds"], 4), setup_s=round(st["setup_seconds"], 4),
+                finish_s=round(st["finish_seconds"], 4), replays=st["replays"],
+                polls=st["polls"], host_fallbacks=st["host_fallbacks"], graph=st["graph"],
+                groups=st["groups"],
+                latency_s={k: round(float(np.mean([a for a, _ in v])), 4)
+                           for k, v in sorted(lat.items())},
+                iterations={k: int(np.mean([b for _, b in v])) for k, v in sorted(lat.items())},
+                count={k: len(v) for k, v in sorted(lat.items())})
+
+
 def oracle_certificate(Ns, gap, pressure, workers):
     """Independent FP64 certificate of a GPU pressure with the oracle operator."""
     from oracle import oracle as O
@@ -121,6 +165,9 @@ def main():
     ap.add_argument("--solve-once", choices=("mixed", "fp64"),
                     help="one cached-operator solve of the given precision (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the 64-instance batched workload")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -133,6 +180,10 @@ def main():
 
     Ns = args.ns
     assert hc.device_count() >= 1, "no GPU visible"
+
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": run_ensemble(hc, DeviceArray, args.steps)}))
+        return
 
     if args.solve_once:
         gap = rough_gap(Ns, H=0.8, k_low=4, k_high=64, rms=0.002, seed=0)
@@ -163,7 +214,13 @@ def main():
         log("[bench] FP%d matvec %.3f ms (%.0f GB/s alg.), stages %s ms" %
             (prec, ms, mv[prec]["gbs"], ["%.3f" % t for t in stages]))
     if args.matvec_only:
-        print(json.dumps({"matvec": mv}))
+        # counter calibration: a plain streaming copy of known size (256 MiB
+        # read + 256 MiB written per launch) by the library's own copy kernel
+        n = 256 * 1024 * 1024 // 8
+        a, b = DeviceArray(n, np.float64), DeviceArray(n, np.float64)
+        for _ in range(3):
+            b.copy_from(a)
+        print(json.dumps({"matvec": mv, "calibration_copy_bytes": n * 8}))
         return
 
     # ── certified solves ───────────────────────────────────────────────────
@@ -208,6 +265,12 @@ def main():
                     matvec_gbs=round(m32["gbs"], 1),
                     matvec_frac=round(m32["gbs"] / HBM_PEAK_GBS, 4),
                     matvec_ms=round(m32["ms"], 4))
+
+    ens = None
+    if not args.no_ensemble:
+        log("[bench] ensemble (64 instances)")
+        ens = run_ensemble(hc, DeviceArray, 1)
+        log("[bench]   %.3f s, %.1f solves/s" % (ens["seconds"], ens["solves_per_s"]))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -262,6 +325,7 @@ def main():
                    "fp64_stage_ms": [round(t, 4) for t in mv[64]["stages_ms"]]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(out))
 

@@ -246,6 +246,53 @@ int aspher_solver_adjoint_device(aspher_solver* s, const aspher_result* res,
                                  const aspher_adjoint_params* prm, double* grad_gap_dev,
                                  aspher_adjoint_info* info);
 
+/* Ensemble: many independent certified solves of uneven grid sizes advanced
+ * together (no reference counterpart; each instance is the solve of
+ * ContactSolver(backend).solve(gap, p_nominal, precond=...),
+ * python/bindings.cpp:186-234, single level). Instances are grouped by
+ * (backend, grid_size, domain_size, E_star); every O(N) pass of an iteration
+ * is one launch over a per-instance work queue, the spectral operators one
+ * batched launch per group, and convergence is decided on the device, so the
+ * iteration is captured in a HIP graph and replayed; the host polls the
+ * instances' flags every `check_every` replays. Tile shapes of the batched
+ * transforms are chosen by an on-device autotuner (cached per process). */
+typedef struct {
+    double tol;              /* 1e-5; FP32 ensembles certify at max(tol, 2e-6) */
+    int max_iter;            /* 5000 */
+    int use_pr;              /* 1 */
+    char precision[16];      /* "float" (FP32 + FP64 reductions) | "double" */
+    char precond[16];        /* "auto" (stencil for fft, Fourier for periodic),
+                                "none", "fourier", "fourier-fft" */
+    int check_every;         /* 8 */
+    int use_graph;           /* 1 */
+    int autotune;            /* 1 */
+    int keep_best;           /* 1 */
+    int stall_limit;         /* 200 */
+    int light_result;        /* 0 */
+} aspher_ensemble_params;
+
+typedef struct {
+    double seconds, setup_seconds, loop_seconds, finish_seconds, tune_seconds;
+    int groups, replays, polls, host_fallbacks, device_iterations, graph;
+} aspher_ensemble_stats;
+
+void aspher_ensemble_params_default(aspher_ensemble_params* p);
+/* n instances; backend[i]: 0 "fft", 1 "periodic"; gaps[i]: Ns_i^2 doubles
+ * (host for aspher_solve_ensemble, device for the _device variant);
+ * results[i] receives a result handle; instance_seconds (may be NULL) the
+ * time from the start of the loop until instance i left the device. */
+int aspher_solve_ensemble(int n, const int* backend, const int* grid_size,
+                          const double* domain_size, const double* E_star,
+                          const double* p_nominal, const double* const* gaps,
+                          const aspher_ensemble_params* params, aspher_result** results,
+                          aspher_ensemble_stats* stats, double* instance_seconds);
+int aspher_solve_ensemble_device(int n, const int* backend, const int* grid_size,
+                                 const double* domain_size, const double* E_star,
+                                 const double* p_nominal, const double* const* gaps_dev,
+                                 const aspher_ensemble_params* params,
+                                 aspher_result** results, aspher_ensemble_stats* stats,
+                                 double* instance_seconds);
+
 /* Device buffers for callers of the *_device entry points that have no
  * allocator of their own (tests, benchmarks). No reference counterpart: the
  * reference is host-only. */
@@ -253,6 +300,10 @@ int aspher_device_malloc(int64_t bytes, void** ptr);
 int aspher_device_free(void* ptr);
 int aspher_memcpy_htod(void* dst, const void* src, int64_t bytes);
 int aspher_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
+/* device-to-device copy by a plain streaming kernel of this library (reads
+ * and writes exactly `bytes`, a multiple of 16): the calibration of the HBM
+ * traffic counters. No reference counterpart. */
+int aspher_copy_device(void* dst, const void* src, int64_t bytes);
 
 #ifdef __cplusplus
 }
