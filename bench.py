@@ -54,6 +54,19 @@ def alg_bytes(Ns: int, s: int):
                 total=rows_fwd + cols + rows_inv)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the counters' reading of a
+    copy of known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", "matvec_fft_ns4096_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes %s = " % kernel):
+            return int(line.split("=")[-1])
+    return None
+
+
 def run_solve(hc, Ns, gptr, precision, reuse, float_floor):
     kw = dict(coarsest=64, backend="fft", precision=precision, light_result=False,
               reuse_operators=reuse)
@@ -258,8 +271,13 @@ def main():
     names = ("rows_fwd", "cols", "rows_inv")
     k = int(np.argmax(m32["stages_ms"]))
     ach = m32["bytes"][names[k]] / (m32["stages_ms"][k] * 1e-3) / 1e9
+    kname = {"rows_fwd": "k_rb_rows_fwd<float, 12, 0>", "cols": "k_rb_cols<float, 13, 0>",
+             "rows_inv": "k_rb_rows_inv<float, 12, 0>"}[names[k]]
+    traffic = profiled_traffic(kname) if Ns == 4096 else None
     roofline = dict(bound="hbm", kernel=names[k], achieved=round(ach, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=("profiles/matvec_fft_ns4096_pmc_traffic.md (%s)" % kname
+                                    if traffic else None),
                     bytes_per_launch=m32["bytes"][names[k]],
                     launch_ms=round(m32["stages_ms"][k], 4),
                     matvec_gbs=round(m32["gbs"], 1),
