@@ -143,6 +143,7 @@ void Operator::build_single(hipStream_t s) {
 void Operator::release_single() {
     plan32_.reset();
     kh32_.reset();
+    khp32_.reset();
     tmp32_.reset();
 }
 
@@ -159,14 +160,11 @@ template <typename T>
 void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
     if (!built_) throw std::logic_error("FFTOperator::matvec: build() not called");
     if (kind_ == Kind::free_space) {
-        const T* kh;
-        if constexpr (std::is_same_v<T, double>) kh = kh64_.p;
-        else {
-            if (!kh32_.p) build_single(s);
-            kh = kh32_.p;
-        }
-        plan<T>(s).convolve(x, Ns_, y, Ns_, Symbol::table, kh, Ns_ + 1, 0.0, s, RowIn::plain,
-                           RowOut::plain, nullptr, nullptr, 0, nullptr, ev);
+        SpectralPlan<T>& pl = plan<T>(s);
+        int kp = 0;
+        const T* kh = table_for<T>(pl, s, kp);
+        pl.convolve(x, Ns_, y, Ns_, Symbol::table, kh, kp, 0.0, s, RowIn::plain, RowOut::plain,
+                    nullptr, nullptr, 0, nullptr, ev);
     } else if (kind_ == Kind::periodic) {
         // q = 2 pi k/L, u_hat = 2 p_hat/(E*|q|), unnormalised pair -> 1/Ns^2
         // (src/periodic_fft_operator.cpp:11-24)
@@ -191,6 +189,37 @@ void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
     }
 }
 
+template <typename T>
+const T* Operator::table_for(const SpectralPlan<T>& pl, hipStream_t s, int& pitch) {
+    if (kind_ != Kind::free_space || !built_)
+        throw std::logic_error("table_for: free-space operator not built");
+    const T* kh;
+    DevBuf<T>* perm;
+    if constexpr (std::is_same_v<T, double>) {
+        kh = kh64_.p;
+        perm = &khp64_;
+    } else {
+        if (!kh32_.p) build_single(s);
+        kh = kh32_.p;
+        perm = &khp32_;
+    }
+    if (!pl.table_permuted()) {
+        pitch = Ns_ + 1;
+        return kh;
+    }
+    if (pl.n != 2 * Ns_ || pl.nh != Ns_ + 1)
+        throw std::invalid_argument("table_for: plan does not match the operator");
+    if (!perm->p) {
+        perm->alloc(pl.table_elems());
+        pl.permute_table(kh, Ns_ + 1, perm->p, s);
+    }
+    pitch = pl.n;
+    return perm->p;
+}
+template const double* Operator::table_for<double>(const SpectralPlan<double>&, hipStream_t,
+                                                   int&);
+template const float* Operator::table_for<float>(const SpectralPlan<float>&, hipStream_t, int&);
+
 void Operator::apply(const double* x, double* y, hipStream_t s) { apply_t<double>(x, y, s, nullptr); }
 void Operator::apply(const float* x, float* y, hipStream_t s) { apply_t<float>(x, y, s, nullptr); }
 void Operator::apply_timed(const double* x, double* y, hipStream_t s, hipEvent_t* ev) {
@@ -201,7 +230,7 @@ void Operator::apply_timed(const float* x, float* y, hipStream_t s, hipEvent_t* 
 }
 
 std::size_t Operator::bytes_spectrum() const {
-    return kh64_.bytes() + kh32_.bytes();
+    return kh64_.bytes() + kh32_.bytes() + khp64_.bytes() + khp32_.bytes();
 }
 std::size_t Operator::bytes_scratch() const {
     std::size_t b = 0;

@@ -142,12 +142,10 @@ public:
     double build_seconds = 0.0;
     // device Love table (dense kind; free space keeps it only if asked)
     const double* table() const { return table_.p; }
-    // free-space kernel spectrum, (Ns+1)^2 [kx][ky] (FP32 copy made on demand)
-    const double* spectrum64() const { return kh64_.p; }
-    const float* spectrum32(hipStream_t s) {
-        if (!kh32_.p) build_single(s);
-        return kh32_.p;
-    }
+    // free-space kernel spectrum in the Symbol::table layout that plan `pl`
+    // reads (made on demand per precision); pitch receives its row pitch
+    template <typename T>
+    const T* table_for(const SpectralPlan<T>& pl, hipStream_t s, int& pitch);
 
 private:
     template <typename T> void apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev);
@@ -160,6 +158,8 @@ private:
     DevBuf<double> table_;
     DevBuf<double> kh64_;      // (Ns+1) x (Ns+1), [kx][ky]
     DevBuf<float> kh32_;
+    DevBuf<double> khp64_;     // (Ns+1) x 2Ns in the register-blocked column order
+    DevBuf<float> khp32_;
     std::unique_ptr<SpectralPlan<double>> plan64_;
     std::unique_ptr<SpectralPlan<float>> plan32_;
     DevBuf<double> tmp64_;     // dense aliasing scratch

@@ -108,11 +108,11 @@ SpectralTuning autotune_plan(int n, int rows_in, int cols_in, int rows_out, int 
     const std::size_t lds_max = 160 * 1024;
     std::vector<int> Bs, Ws;
     for (int B = 1; B <= 64; B *= 2)
-        if (B * (L / 16) <= 512 && B * cs * (std::size_t)(L + L / 32 + 1) <= lds_max &&
+        if (B * (L / 16) <= 512 && B * cs * (std::size_t)(L + L / 16 + 1) <= lds_max &&
             B <= std::max(rows_in, rows_out))
             Bs.push_back(B);
     for (int W = 1; W <= 64; W *= 2)
-        if (W * (n / 16) <= 512 && W * cs * (std::size_t)(n + n / 32 + 1) <= lds_max &&
+        if (W * (n / 16) <= 512 && W * cs * (std::size_t)(n + n / 16 + 1) <= lds_max &&
             W <= n / 2 + 1)
             Ws.push_back(W);
     if (Bs.empty() || Ws.empty()) return SpectralTuning{};
@@ -123,9 +123,10 @@ SpectralTuning autotune_plan(int n, int rows_in, int cols_in, int rows_out, int 
     DevBuf<T> kt;
     int kp = 0;
     if (sym == Symbol::table) {
-        kp = n / 2 + 1;
-        kt.alloc((std::size_t)kp * kp);
-        VecOps<T>::fill((std::int64_t)kp * kp, kt.p, 1e-3, s);
+        // the register-blocked layout: n/2+1 rows of n values
+        kp = n;
+        kt.alloc((std::size_t)(n / 2 + 1) * n);
+        VecOps<T>::fill((std::int64_t)(n / 2 + 1) * n, kt.p, 1e-3, s);
     }
     SpectralTuning best{};
     double t_best_rows = 1e30, t_best_cols = 1e30;
@@ -186,6 +187,7 @@ struct Group {
     DevBuf<WorkItem> tiles;                        // stencil work queue
     int ntiles = 0;
     const Real* ktab = nullptr;
+    int kpitch = 0;
     double sym_scale = 0.0;
 };
 
@@ -343,8 +345,7 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
         if (opt.autotune) tn = autotune_plan<Real>(n, g.Ns, g.Ns, g.Ns, g.Ns, g.count, sym, s);
         g.plan = std::make_unique<SpectralPlan<Real>>(n, g.Ns, g.Ns, g.Ns, g.Ns, tn, g.count);
         if (g.kind == 0) {
-            if constexpr (is_double) g.ktab = g.op->spectrum64();
-            else g.ktab = g.op->spectrum32(s);
+            g.ktab = g.op->template table_for<Real>(*g.plan, s, g.kpitch);
         } else {
             const double pi = std::acos(-1.0);
             g.sym_scale = g.L / (pi * g.E * (double)g.Ns * g.Ns);
@@ -425,7 +426,7 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
             sb.gate = S_ACTIVE;
             const Symbol sym = g.kind == 0 ? Symbol::table : Symbol::periodic;
             g.plan->convolve(x + off[g.first], g.Ns, y + off[g.first], g.Ns, sym, g.ktab,
-                             g.Ns + 1, g.sym_scale, s, RowIn::plain, RowOut::plain, nullptr,
+                             g.kpitch, g.sym_scale, s, RowIn::plain, RowOut::plain, nullptr,
                              nullptr, 0, nullptr, nullptr, &sb);
         }
     };

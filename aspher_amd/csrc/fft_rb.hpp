@@ -22,6 +22,8 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace aspher {
 namespace rb {
 
@@ -114,6 +116,50 @@ __device__ __forceinline__ void dft16(cx<T>* a) {
     }
 }
 
+// dft16 with a[8..15] == 0 (the zero-padded half of a free-space
+// transform): the first 4-point pass sees two zero inputs.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16_zin(cx<T>* a) {
+    cx<T> b[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        const cx<T> x0 = a[n2], x1 = a[4 + n2];
+        const cx<T> r1 = rot90<T, INV>(x1);
+        const cx<T> t[4] = {x0 + x1, x0 + r1, x0 - x1, x0 - r1};
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) b[4 * k1 + n2] = w32<T, INV>(t[k1], 2 * k1 * n2);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cx<T> t[4] = {b[4 * k1], b[4 * k1 + 1], b[4 * k1 + 2], b[4 * k1 + 3]};
+        dft4<T, INV>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = t[k2];
+    }
+}
+
+// dft16 computing only the outputs a[0..7] (the half of a free-space
+// transform that is kept); a[8..15] are left unspecified.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16_hout(cx<T>* a) {
+    cx<T> b[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cx<T> t[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+        dft4<T, INV>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) b[4 * k1 + n2] = w32<T, INV>(t[k1], 2 * k1 * n2);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        const cx<T> s0 = b[4 * k1] + b[4 * k1 + 2], d0 = b[4 * k1] - b[4 * k1 + 2];
+        const cx<T> s1 = b[4 * k1 + 1] + b[4 * k1 + 3];
+        const cx<T> d1 = rot90<T, INV>(b[4 * k1 + 1] - b[4 * k1 + 3]);
+        a[k1] = s0 + s1;
+        a[k1 + 4] = d0 + d1;
+    }
+}
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dft(cx<T>* a) {
     if constexpr (R == 1) {
@@ -157,38 +203,65 @@ struct Shape {
     }
 };
 
-// LDS index with a skew of one element every 32 (breaks power-of-two bank
-// strides of the exchanges)
-__device__ __forceinline__ int lds_ix(int p) { return p + (p >> 5); }
-constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
+// LDS index with a skew of one element every 16. The exchanges read and
+// write with lane strides of 1, 16 and 32 elements; with this skew those
+// strides map 16 lanes of 8-byte elements (and 8 lanes of 16-byte elements)
+// to distinct banks (a skew of one per 32 left two-way conflicts on the
+// stride-32 exchanges).
+__device__ __forceinline__ int lds_ix(int p) { return p + (p >> 4); }
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 4) + 1; }
 
-// twiddles W^{v*idx}, v = 0..N-1, applied to a group: one FP64 table read
-// (tw[t] = exp(-2 pi i t/len)), powers as a running FP64 product (at most 15
-// roundings, ~1e-15 relative), rounded once to T. One table read per group
-// measured faster than reading every power (the table reads, not the FP64
-// arithmetic, bound the stage).
+// twiddles W^{v*idx}, v = 0..N-1, applied to a group from one FP64 table
+// read w = tw[idx] (tw[t] = exp(-2 pi i t/len)).
+// FP64 data: powers as a running FP64 product (at most 15 roundings,
+// ~1e-15 relative). FP32 data: w, w^2, w^4, w^8 by FP64 squaring, rounded
+// once to FP32; the other powers are FP32 products of at most three of
+// those (<= 4 roundings, ~2.5e-7 relative, the accuracy of an FP32 twiddle
+// table). One table read per group measured faster than reading every power
+// (the table reads, not the arithmetic, bound the stage).
 template <typename T, int NS, bool INV>
 __device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<double>* __restrict__ tw,
                                               int idx, int /*tmask*/) {
     if (idx == 0) return;
     cx<double> w = tw[idx];
     if (INV) w.im = -w.im;
-    cx<double> wv = w;
+    if constexpr (std::is_same_v<T, float> && NS == 16) {
+        const cx<double> w2 = w * w, w4 = w2 * w2, w8 = w4 * w4;
+        auto f32 = [](cx<double> z) { return cx<float>{(float)z.re, (float)z.im}; };
+        cx<float> f[16];
+        f[1] = f32(w);
+        f[2] = f32(w2);
+        f[4] = f32(w4);
+        f[8] = f32(w8);
+        f[3] = f[2] * f[1];
+        f[5] = f[4] * f[1];
+        f[6] = f[4] * f[2];
+        f[7] = f[4] * f[3];
 #pragma unroll
-    for (int v = 1; v < NS; ++v) {
-        if (v > 1) wv = wv * w;
-        a[v] = a[v] * cx<T>{static_cast<T>(wv.re), static_cast<T>(wv.im)};
+        for (int v = 9; v < 16; ++v) f[v] = f[8] * f[v - 8];
+#pragma unroll
+        for (int v = 1; v < 16; ++v) a[v] = a[v] * f[v];
+    } else {
+        cx<double> wv = w;
+#pragma unroll
+        for (int v = 1; v < NS; ++v) {
+            if (v > 1) wv = wv * w;
+            a[v] = a[v] * cx<T>{static_cast<T>(wv.re), static_cast<T>(wv.im)};
+        }
     }
 }
 
 // Stage s of the forward (INV=false, DFT then twiddle) or inverse (INV=true,
-// twiddle then DFT) transform on the registers of thread th.
-template <typename T, int LOGN, int s, bool INV>
+// twiddle then DFT) transform on the registers of thread th. PR (stage 0
+// only, where every thread holds one group of 16 at positions v*S_0 + th):
+// 1 = inputs v >= 8 are zero, 2 = only outputs v < 8 are needed.
+template <typename T, int LOGN, int s, bool INV, int PR = 0>
 __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __restrict__ tw,
                                       int twmul) {
     using Sh = Shape<LOGN>;
     constexpr int Ns = Sh::N(s), Ss = Sh::S(s), Gs = Sh::G(s);
     constexpr int tstep = Sh::n / (Ns * Ss);     // table stride of W_{Ns Ss}
+    static_assert(PR == 0 || (s == 0 && Ns == 16 && Gs == 1), "pruned stage 0 only");
 #pragma unroll
     for (int q = 0; q < Gs; ++q) {
         cx<T>* a = r + q * Ns;
@@ -196,9 +269,11 @@ __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __rest
         if constexpr (INV) {
             if constexpr (s < Sh::m - 1)
                 twiddle_group<T, Ns, true>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
-            dft<T, Ns, true>(a);
+            if constexpr (PR == 2) dft16_hout<T, true>(a);
+            else dft<T, Ns, true>(a);
         } else {
-            dft<T, Ns, false>(a);
+            if constexpr (PR == 1) dft16_zin<T, false>(a);
+            else dft<T, Ns, false>(a);
             if constexpr (s < Sh::m - 1)
                 twiddle_group<T, Ns, false>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
         }
@@ -222,37 +297,39 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th) {
         for (int v = 0; v < Nb; ++v) r[q * Nb + v] = lds[lds_ix(Sh::pos(sb, th * Gb + q, v))];
 }
 
-template <typename T, int LOGN, int s>
+template <typename T, int LOGN, int s, bool ZIN>
 __device__ __forceinline__ void forward_from(cx<T>* r, cx<T>* lds, int th,
                                              const cx<double>* __restrict__ tw, int twmul) {
-    stage<T, LOGN, s, false>(r, th, tw, twmul);
+    stage<T, LOGN, s, false, (s == 0 && ZIN) ? 1 : 0>(r, th, tw, twmul);
     if constexpr (s + 1 < Shape<LOGN>::m) {
         exchange<T, LOGN, s, s + 1>(r, lds, th);
-        forward_from<T, LOGN, s + 1>(r, lds, th, tw, twmul);
+        forward_from<T, LOGN, s + 1, ZIN>(r, lds, th, tw, twmul);
     }
 }
 
-template <typename T, int LOGN, int s>
+template <typename T, int LOGN, int s, bool HOUT>
 __device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
                                              const cx<double>* __restrict__ tw, int twmul) {
-    stage<T, LOGN, s, true>(r, th, tw, twmul);
+    stage<T, LOGN, s, true, (s == 0 && HOUT) ? 2 : 0>(r, th, tw, twmul);
     if constexpr (s > 0) {
         exchange<T, LOGN, s, s - 1>(r, lds, th);
-        inverse_from<T, LOGN, s - 1>(r, lds, th, tw, twmul);
+        inverse_from<T, LOGN, s - 1, HOUT>(r, lds, th, tw, twmul);
     }
 }
 
 // forward: registers in stage-0 grouping (natural order) -> stage m-1
-// grouping (digit-reversed); inverse: the reverse.
-template <typename T, int LOGN>
+// grouping (digit-reversed); inverse: the reverse. ZIN: the upper half of
+// the input (positions >= n/2, i.e. v >= 8 of stage 0) is zero; HOUT: only
+// the lower half of the output is used.
+template <typename T, int LOGN, bool ZIN = false>
 __device__ __forceinline__ void forward(cx<T>* r, cx<T>* lds, int th,
                                         const cx<double>* __restrict__ tw, int twmul) {
-    forward_from<T, LOGN, 0>(r, lds, th, tw, twmul);
+    forward_from<T, LOGN, 0, ZIN>(r, lds, th, tw, twmul);
 }
-template <typename T, int LOGN>
+template <typename T, int LOGN, bool HOUT = false>
 __device__ __forceinline__ void inverse(cx<T>* r, cx<T>* lds, int th,
                                         const cx<double>* __restrict__ tw, int twmul) {
-    inverse_from<T, LOGN, Shape<LOGN>::m - 1>(r, lds, th, tw, twmul);
+    inverse_from<T, LOGN, Shape<LOGN>::m - 1, HOUT>(r, lds, th, tw, twmul);
 }
 
 } // namespace rb

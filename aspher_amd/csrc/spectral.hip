@@ -365,8 +365,8 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
     if (rb_) {
         // threads per row/column are L/16 resp. n/16; aim for 512-thread
         // workgroups while the LDS of one workgroup stays <= 96 KiB
-        const std::size_t rowlds = cs * (std::size_t)(Lr + Lr / 32 + 1);
-        const std::size_t collds = cs * (std::size_t)(n + n / 32 + 1);
+        const std::size_t rowlds = cs * (std::size_t)(Lr + Lr / 16 + 1);
+        const std::size_t collds = cs * (std::size_t)(n + n / 16 + 1);
         int B = tune.row_batch > 0 ? tune.row_batch : std::max(1, 512 / (Lr / 16));
         while (B > 1 && (B * (Lr / 16) > 512 || B * rowlds > 98304)) --B;
         int W = tune.col_width > 0 ? tune.col_width : std::max(1, 512 / (n / 16));
@@ -423,6 +423,12 @@ std::size_t SpectralPlan<T>::workspace_bytes() const {
     return sizeof(cx<T>) * a_rows_ * pitch * batch_ +
            (rb_ ? sizeof(cx<double>) * n + sizeof(cx<T>) * (std::size_t)nh * pitchT_ * batch_
                 : sizeof(cx<T>) * n);
+}
+
+template <typename T>
+void SpectralPlan<T>::permute_table(const T* kunique, int kp, T* kout, hipStream_t s) const {
+    if (!rb_) throw std::logic_error("permute_table: generic plans read the unique table");
+    rb_permute_table<T>(logn_, kunique, kp, nh, kout, s);
 }
 
 template <typename T>

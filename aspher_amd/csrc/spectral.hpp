@@ -102,6 +102,12 @@ public:
 
     int row_out_blocks() const;   // blocks of rows_inv (partial count)
     int batch_capacity() const { return batch_; }
+    // Symbol::table layout the kernels of this plan read: the register-
+    // blocked path wants the spectrum in its column register order (nh rows
+    // of n values, pitch n); the generic path the unique (n/2+1)-wide table.
+    bool table_permuted() const { return rb_; }
+    std::size_t table_elems() const { return (std::size_t)nh * n; }
+    void permute_table(const T* kunique, int kp, T* kout, hipStream_t s) const;
     SpectralTuning tuning() const { return {row_batch_, col_width_, xcd_remap_}; }
     int col_blocks() const;       // blocks of cols (partial count, build)
     std::size_t workspace_bytes() const;

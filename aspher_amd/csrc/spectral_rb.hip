@@ -15,6 +15,7 @@
 #include "reduce.hpp"
 #include "spectral_rb.hpp"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -29,7 +30,10 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
 }
 
 // ── rows: real -> half spectrum (transposed store) ─────────────────────────
-template <typename T, int LOGL, int IN>
+// Z: the row's nonzero reals are x[0 .. L) of n = 2L (zero-padded free-space
+// rows), so the upper half of the packed input is never loaded and the first
+// radix-16 stage is pruned.
+template <typename T, int LOGL, int IN, bool Z>
 __global__ void __launch_bounds__(512)
 k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
               cx<T>* __restrict__ AT, int pitchT, int B, const cx<double>* __restrict__ tw,
@@ -75,9 +79,10 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const int p = Sh::pos(0, th, v);   // packed index j: reals 2j, 2j+1
-        r[v] = {load(2 * p), load(2 * p + 1)};
+        if (Z && v >= 8) r[v] = {T(0), T(0)};
+        else r[v] = {load(2 * p), load(2 * p + 1)};
     }
-    rb::forward<T, LOGL>(r, lds, th, tw, 2);
+    rb::forward<T, LOGL, Z>(r, lds, th, tw, 2);
     // digit-reversed registers -> natural order in LDS, then split the
     // packed spectrum: X[k] = (Z_k + conj Z_{L-k})/2 - i W^k (Z_k - conj Z_{L-k})/2
     __syncthreads();
@@ -108,14 +113,34 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
     }
 }
 
+// 16 contiguous values from a 64-byte (FP32) / 128-byte (FP64) aligned
+// address as 16-byte vector loads
+__device__ __forceinline__ void load16(const float* __restrict__ p, float* v) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float4 t = q[j];
+        v[4 * j] = t.x;
+        v[4 * j + 1] = t.y;
+        v[4 * j + 2] = t.z;
+        v[4 * j + 3] = t.w;
+    }
+}
+__device__ __forceinline__ void load16(const double* __restrict__ p, double* v) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double2 t = q[j];
+        v[2 * j] = t.x;
+        v[2 * j + 1] = t.y;
+    }
+}
+
 // ── columns: forward . symbol . inverse (transposed in, row-major out) ─────
 template <typename T, int SYM>
 __device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restrict__ ktab,
                                        int kp, double sc) {
-    if constexpr (SYM == 0) {
-        const int kyp = (ky <= (n >> 1)) ? ky : n - ky;
-        return ktab[(size_t)kx * kp + kyp];
-    } else if constexpr (SYM == 1) {
+    if constexpr (SYM == 1) {
         const int my = (ky < ((n + 1) >> 1)) ? ky : ky - n;
         const double m2 = (double)kx * kx + (double)my * my;
         return m2 > 0.0 ? static_cast<T>(sc / sqrt(m2)) : T(0);
@@ -151,14 +176,18 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
     const bool live = kx < ncols;
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem) + c * LL;
 
+    // free-space symbol: rows_in, rows_out <= n/2, so the zero-padded upper
+    // half is neither loaded nor transformed forward, nor produced inverse
+    constexpr bool ZP = (SYM == 0);
     cx<T> r[16];
     const cx<T>* col = AT + (size_t)(live ? kx : 0) * pitchT;
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const int p = Sh::pos(0, th, v);
-        r[v] = (live && p < rows_in) ? col[p] : cx<T>{T(0), T(0)};
+        if (ZP && v >= 8) r[v] = {T(0), T(0)};
+        else r[v] = (live && p < rows_in) ? col[p] : cx<T>{T(0), T(0)};
     }
-    rb::forward<T, LOGN>(r, lds, th, tw, 1);
+    rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1);
 
     if constexpr (SYM == 3) {
         // kernel-spectrum build (src/fft_operator.cpp:46-68): realness check
@@ -183,18 +212,28 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
         if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = mr;
         return;
     } else {
+        if constexpr (SYM == 0) {
+            // table in the plan's digit-reversed layout: the 16 values of
+            // this thread are contiguous at kx*kp + 16*th (Gl*Nl = 16)
+            const T* kr = ktab + (size_t)(live ? kx : 0) * kp + 16 * th;
+            T kv[16];
+            load16(kr, kv);
 #pragma unroll
-        for (int q = 0; q < Gl; ++q)
+            for (int i = 0; i < 16; ++i) r[i] = scale(r[i], live ? kv[i] : T(0));
+        } else {
 #pragma unroll
-            for (int v = 0; v < Nl; ++v) {
-                const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
-                const T s = live ? symbol_rb<T, SYM>(kx, ky, n, ktab, kp, sc) : T(0);
-                r[q * Nl + v] = scale(r[q * Nl + v], s);
-            }
-        rb::inverse<T, LOGN>(r, lds, th, tw, 1);
+            for (int q = 0; q < Gl; ++q)
+#pragma unroll
+                for (int v = 0; v < Nl; ++v) {
+                    const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+                    const T s = live ? symbol_rb<T, SYM>(kx, ky, n, ktab, kp, sc) : T(0);
+                    r[q * Nl + v] = scale(r[q * Nl + v], s);
+                }
+        }
+        rb::inverse<T, LOGN, ZP>(r, lds, th, tw, 1);
         if (live) {
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
+            for (int v = 0; v < (ZP ? 8 : 16); ++v) {
                 const int p = Sh::pos(0, th, v);
                 if (p < rows_out) BR[(size_t)p * pitch + kx] = r[v];
             }
@@ -203,7 +242,9 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
 }
 
 // ── rows: half spectrum (row-major) -> real ────────────────────────────────
-template <typename T, int LOGL, int OUT>
+// H: only the reals y[0 .. L) of n = 2L are wanted (free-space rows), so the
+// last radix-16 stage computes half of its outputs.
+template <typename T, int LOGL, int OUT, bool H>
 __global__ void __launch_bounds__(512)
 k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
               const cx<double>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
@@ -255,11 +296,11 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
             const cx<T> D = (Xk - Xc) * cx<T>{static_cast<T>(wd.re), static_cast<T>(-wd.im)};
             r[q * Nl + v] = {S.re - D.im, S.im + D.re};
         }
-    rb::inverse<T, LOGL>(r, lds, th, tw, 2);
+    rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
     double acc = 0.0;
     if (row < nrows) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
+        for (int v = 0; v < (H ? 8 : 16); ++v) {
             const int p = Sh::pos(0, th, v);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -302,18 +343,22 @@ void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
     const void* fn;
 #define RF_ARGS a.x, a.x_pitch, a.ncols, a.nrows, a.AT, a.pitchT, a.B, a.tw, a.mask, a.scal, \
         a.gmean_idx, a.table, a.Ns_embed, a.remap, a.bt
-    if (a.mode == 0) {
-        fn = (const void*)k_rb_rows_fwd<T, LOGL, 0>;
+    if (a.mode == 0 && a.ncols <= L) {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 0, true>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_fwd<T, LOGL, 0><<<grid, block, lds, s>>>(RF_ARGS);
+        k_rb_rows_fwd<T, LOGL, 0, true><<<grid, block, lds, s>>>(RF_ARGS);
+    } else if (a.mode == 0) {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 0, false>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_fwd<T, LOGL, 0, false><<<grid, block, lds, s>>>(RF_ARGS);
     } else if (a.mode == 1) {
-        fn = (const void*)k_rb_rows_fwd<T, LOGL, 1>;
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 1, false>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_fwd<T, LOGL, 1><<<grid, block, lds, s>>>(RF_ARGS);
+        k_rb_rows_fwd<T, LOGL, 1, false><<<grid, block, lds, s>>>(RF_ARGS);
     } else {
-        fn = (const void*)k_rb_rows_fwd<T, LOGL, 2>;
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 2, false>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_fwd<T, LOGL, 2><<<grid, block, lds, s>>>(RF_ARGS);
+        k_rb_rows_fwd<T, LOGL, 2, false><<<grid, block, lds, s>>>(RF_ARGS);
     }
 #undef RF_ARGS
     ASPHER_LAUNCH_CHECK();
@@ -334,6 +379,8 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
         k_rb_cols<T, LOGN, S><<<grid, block, lds, s>>>(RC_ARGS);       \
         break;                                                         \
     }
+    if (a.sym == 0 && (a.rows_in > n / 2 || a.rows_out > n / 2))
+        throw std::invalid_argument("table symbol needs rows_in, rows_out <= n/2");
     switch (a.sym) {
         RC_CASE(0)
         RC_CASE(1)
@@ -354,19 +401,50 @@ void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
 #define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap, \
         a.bt
-    if (a.mode == 0) {
-        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0>;
+    if (a.mode == 0 && a.ncols <= L) {
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, true>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_inv<T, LOGL, 0><<<grid, block, lds, s>>>(RI_ARGS);
+        k_rb_rows_inv<T, LOGL, 0, true><<<grid, block, lds, s>>>(RI_ARGS);
+    } else if (a.mode == 0) {
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, false>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_inv<T, LOGL, 0, false><<<grid, block, lds, s>>>(RI_ARGS);
     } else {
-        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 1>;
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 1, false>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_inv<T, LOGL, 1><<<grid, block, lds, s>>>(RI_ARGS);
+        k_rb_rows_inv<T, LOGL, 1, false><<<grid, block, lds, s>>>(RI_ARGS);
     }
 #undef RI_ARGS
     ASPHER_LAUNCH_CHECK();
 }
 
+
+// ── free-space kernel spectrum in the column kernel's register order ──────
+// kout[kx*n + pos] = K(kx, ky) with ky = kidx(pos) the frequency that the
+// forward column transform leaves at digit-reversed position pos, folded by
+// evenness to the (n/2+1)-wide unique table kin[kx*kp + min(ky, n-ky)].
+template <typename T, int LOGN>
+__global__ void __launch_bounds__(256)
+k_rb_permute_table(const T* __restrict__ kin, int kp, int nh, T* __restrict__ kout) {
+    using Sh = rb::Shape<LOGN>;
+    constexpr int n = Sh::n;
+    const std::int64_t total = (std::int64_t)nh * n;
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i / n), pos = (int)(i - (std::int64_t)kx * n);
+        const int ky = Sh::kidx(pos);
+        const int kyp = (ky <= (n >> 1)) ? ky : n - ky;
+        kout[i] = kin[(std::size_t)kx * kp + kyp];
+    }
+}
+
+template <typename T, int LOGN>
+void launch_permute(const T* kin, int kp, int nh, T* kout, hipStream_t s) {
+    const std::int64_t total = (std::int64_t)nh * (1 << LOGN);
+    const int grid = (int)std::min<std::int64_t>((total + 255) / 256, 65536);
+    k_rb_permute_table<T, LOGN><<<grid, 256, 0, s>>>(kin, kp, nh, kout);
+    ASPHER_LAUNCH_CHECK();
+}
 } // namespace
 
 template <typename T>
@@ -418,6 +496,24 @@ void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s) {
         default: throw std::invalid_argument("row transform length out of range");
     }
 }
+
+template <typename T>
+void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s) {
+    switch (logN) {
+        case 5: launch_permute<T, 5>(kin, kp, nh, kout, s); break;
+        case 6: launch_permute<T, 6>(kin, kp, nh, kout, s); break;
+        case 7: launch_permute<T, 7>(kin, kp, nh, kout, s); break;
+        case 8: launch_permute<T, 8>(kin, kp, nh, kout, s); break;
+        case 9: launch_permute<T, 9>(kin, kp, nh, kout, s); break;
+        case 10: launch_permute<T, 10>(kin, kp, nh, kout, s); break;
+        case 11: launch_permute<T, 11>(kin, kp, nh, kout, s); break;
+        case 12: launch_permute<T, 12>(kin, kp, nh, kout, s); break;
+        case 13: launch_permute<T, 13>(kin, kp, nh, kout, s); break;
+        default: throw std::invalid_argument("column transform length out of range");
+    }
+}
+template void rb_permute_table<float>(int, const float*, int, int, float*, hipStream_t);
+template void rb_permute_table<double>(int, const double*, int, int, double*, hipStream_t);
 
 template void rb_rows_fwd<float>(int, const RbRowsFwd<float>&, hipStream_t);
 template void rb_rows_fwd<double>(int, const RbRowsFwd<double>&, hipStream_t);

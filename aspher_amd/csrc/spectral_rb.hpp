@@ -76,5 +76,9 @@ struct RbRowsInv {
 template <typename T> void rb_rows_fwd(int logL, const RbRowsFwd<T>& a, hipStream_t s);
 template <typename T> void rb_cols(int logN, const RbCols<T>& a, hipStream_t s);
 template <typename T> void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s);
+// unique (n/2+1)^2 free-space spectrum -> nh x n table in the column
+// kernel's register order (see k_rb_permute_table)
+template <typename T>
+void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s);
 
 } // namespace aspher
