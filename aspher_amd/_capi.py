@@ -29,6 +29,7 @@ EXPORTS = (
     "aspher_solver_create_shared", "aspher_adjoint_params_default",
     "aspher_solver_adjoint", "aspher_solver_adjoint_device", "aspher_copy_device",
     "aspher_ensemble_params_default", "aspher_solve_ensemble", "aspher_solve_ensemble_device",
+    "aspher_rough_surface", "aspher_rough_surface_device",
 )
 
 
@@ -155,6 +156,8 @@ def _declare(L):
         "aspher_solver_create_shared": (i, [i, d, d, s, C.POINTER(vp)]),
         "aspher_copy_device": (i, [vp, vp, C.c_int64]),
         "aspher_ensemble_params_default": (None, [C.POINTER(EnsembleParams)]),
+        "aspher_rough_surface": (i, [i, d, d, d, d, C.c_uint64, dp]),
+        "aspher_rough_surface_device": (i, [i, d, d, d, d, C.c_uint64, vp]),
         "aspher_solve_ensemble": (i, [i, ip, ip, dp, dp, dp, C.POINTER(dp),
                                       C.POINTER(EnsembleParams), C.POINTER(vp),
                                       C.POINTER(EnsembleStats), dp]),

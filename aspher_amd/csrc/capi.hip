@@ -793,6 +793,32 @@ int aspher_solve_ensemble_device(int n, const int* backend, const int* grid_size
     });
 }
 
+int aspher_rough_surface(int grid_size, double hurst, double k_low, double k_high, double rms,
+                         uint64_t seed, double* out) {
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null output");
+        if (grid_size <= 0) throw std::invalid_argument("grid_size must be positive");
+        const std::size_t N = (std::size_t)grid_size * grid_size;
+        Context ctx;
+        DevBuf<double> d(N);
+        rough_surface_device(grid_size, hurst, k_low, k_high, rms, seed, d.p, ctx.stream);
+        ASPHER_HIP(hipMemcpyAsync(out, d.p, sizeof(double) * N, hipMemcpyDeviceToHost,
+                                  ctx.stream));
+        ASPHER_HIP(hipStreamSynchronize(ctx.stream));
+    });
+}
+
+int aspher_rough_surface_device(int grid_size, double hurst, double k_low, double k_high,
+                                double rms, uint64_t seed, double* out_dev) {
+    return guard([&] {
+        if (!out_dev) throw std::invalid_argument("null output");
+        ASPHER_HIP(hipDeviceSynchronize());
+        Context ctx;
+        rough_surface_device(grid_size, hurst, k_low, k_high, rms, seed, out_dev, ctx.stream);
+        ASPHER_HIP(hipStreamSynchronize(ctx.stream));
+    });
+}
+
 int aspher_device_malloc(int64_t bytes, void** ptr) {
     return guard([&] {
         if (!ptr || bytes <= 0) throw std::invalid_argument("bad allocation request");

@@ -47,6 +47,7 @@ inline double since(Clock::time_point t0) {
 }
 
 const int kSum1[1] = {RSUM};
+const int kSum2[2] = {RSUM, RSUM};
 
 } // namespace
 
@@ -237,6 +238,38 @@ std::size_t Operator::bytes_scratch() const {
     if (plan64_) b += plan64_->workspace_bytes();
     if (plan32_) b += plan32_->workspace_bytes();
     return b;
+}
+
+// ── Rough-surface generator ────────────────────────────────────────────────
+
+void rough_surface_device(int Ns, double hurst, double k_low, double k_high, double rms,
+                          unsigned long long seed, double* out, hipStream_t s) {
+    if (Ns < 32 || Ns > 8192 || (Ns & (Ns - 1)))
+        throw std::invalid_argument("rough_surface: Ns must be a power of two in [32, 8192]");
+    if (!(rms > 0.0) || !std::isfinite(hurst) || !(k_high >= k_low))
+        throw std::invalid_argument("rough_surface: need rms > 0 and k_low <= k_high");
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    SpectralPlan<double> pl(Ns, Ns, Ns, Ns, Ns);
+    pl.generate_surface(hurst, k_low, k_high, 1.0 / (double)N, seed, out, s);
+    // mean removal, then population std after centring (numpy h.std())
+    const int nb = vec_blocks(N);
+    DevBuf<double> scal(S_COUNT), part((std::size_t)nb * kPartVals);
+    static const int d2[2] = {S_R0, S_R1};
+    double h[S_COUNT];
+    auto moments = [&]() {
+        VecOps<double>::moments(N, out, part.p, nb, s);
+        finalize<double>(part.p, nb, 2, kSum2, d2, scal.p, STEP_NONE, s);
+        ASPHER_HIP(hipMemcpyAsync(h, scal.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        ASPHER_HIP(hipStreamSynchronize(s));
+    };
+    moments();
+    VecOps<double>::shift(N, out, h[S_R0] / (double)N, out, s);
+    moments();
+    const double sd = std::sqrt(h[S_R1] / (double)N);
+    if (!(sd > 0.0))
+        throw std::invalid_argument("rough_surface: empty band (no wavenumber in [k_low, k_high])");
+    VecOps<double>::scale(N, out, rms / sd, s);
+    ASPHER_HIP(hipStreamSynchronize(s));
 }
 
 // ── Preconditioners ────────────────────────────────────────────────────────

@@ -290,6 +290,14 @@ struct AdjointInfo {
 void adjoint_impl(Context& ctx, Operator& op, Precond* pc, const double* p, const double* w,
                   double* grad, const AdjointOptions& opt, AdjointInfo& info);
 
+// Self-affine Gaussian rough surface on the device (the benchmark recipe:
+// random phase, amplitude |m|^-(1+H) for k_low <= |m| <= k_high, real part
+// of the inverse 2-D DFT, mean removed, scaled to rms), with the phase of
+// frequency (kx, ky) drawn as SplitMix64 output number ky*Ns + kx of a
+// generator seeded with `seed`. out: Ns^2 FP64 (device), row-major.
+void rough_surface_device(int Ns, double hurst, double k_low, double k_high, double rms,
+                          unsigned long long seed, double* out, hipStream_t s);
+
 // Process-wide operator cache (kind, Ns, L, E) -> built operator.
 std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L, double E,
                                           hipStream_t s);

@@ -391,6 +391,20 @@ k_sum(std::int64_t N, const T* __restrict__ y, double* __restrict__ part) {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
+k_moments(std::int64_t N, const T* __restrict__ y, double* __restrict__ part) {
+    __shared__ double red[8];
+    double s = 0.0, q = 0.0;
+    GRID_LOOP(i, N) {
+        const double v = (double)y[i];
+        s += v;
+        q += v * v;
+    }
+    put<RSUM>(s, red, part, 2, 0);
+    put<RSUM>(q, red, part, 2, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
 k_count_above(std::int64_t N, const T* __restrict__ y, double theta,
               double* __restrict__ part) {
     __shared__ double red[8];
@@ -1027,6 +1041,10 @@ void VecOps<T>::axpy_id(std::int64_t N, const T* p, const T* g, T* z, double s_i
 template <typename T>
 void VecOps<T>::sum(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s) {
     L1(k_sum<T>, N, y, part);
+}
+template <typename T>
+void VecOps<T>::moments(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s) {
+    L1(k_moments<T>, N, y, part);
 }
 template <typename T>
 void VecOps<T>::count_above(std::int64_t N, const T* y, double theta, double* part,

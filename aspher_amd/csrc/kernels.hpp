@@ -90,6 +90,8 @@ template <typename T> struct VecOps {
     static void axpy_id(std::int64_t N, const T* p, const T* g, T* z, double s_id,
                         hipStream_t st);
     static void sum(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s);
+    // {sum y, sum y^2}
+    static void moments(std::int64_t N, const T* y, double* part, int nblk, hipStream_t s);
     // {sum_{y>theta} y, count_{y>theta}}
     static void count_above(std::int64_t N, const T* y, double theta, double* part,
                             int nblk, hipStream_t s);

@@ -14,6 +14,8 @@
 #include "spectral.hpp"
 #include "spectral_rb.hpp"
 
+#include <type_traits>
+
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -423,6 +425,42 @@ std::size_t SpectralPlan<T>::workspace_bytes() const {
     return sizeof(cx<T>) * a_rows_ * pitch * batch_ +
            (rb_ ? sizeof(cx<double>) * n + sizeof(cx<T>) * (std::size_t)nh * pitchT_ * batch_
                 : sizeof(cx<T>) * n);
+}
+
+template <typename T>
+void SpectralPlan<T>::generate_surface(double hurst, double k_low, double k_high, double scale,
+                                       unsigned long long seed, T* out, hipStream_t s) {
+    if constexpr (!std::is_same_v<T, double>) {
+        throw std::logic_error("generate_surface: FP64 plans only");
+    } else {
+        if (!rb_ || rows_in != n || rows_out != n || cols_in != n || cols_out != n)
+            throw std::invalid_argument(
+                "generate_surface: needs a power-of-two n x n plan, 32 <= n <= 8192");
+        RbGen g;
+        g.BR = A_;
+        g.pitch = pitch;
+        g.hurst = hurst;
+        g.k_low = k_low;
+        g.k_high = k_high;
+        g.scale = scale;
+        g.seed = seed;
+        g.tw = tw64_->dev;
+        g.W = col_width_;
+        g.remap = xcd_remap_ ? 1 : 0;
+        rb_gen_cols(logn_, g, s);
+        RbRowsInv<double> r;
+        r.BR = A_;
+        r.pitch = pitch;
+        r.nrows = n;
+        r.B = row_batch_;
+        r.tw = tw64_->dev;
+        r.y = out;
+        r.y_pitch = n;
+        r.ncols = n;
+        r.mode = 0;
+        r.remap = g.remap;
+        rb_rows_inv<double>(ilog2(n / 2), r, s);
+    }
 }
 
 template <typename T>

@@ -100,6 +100,12 @@ public:
     void build_spectrum(const double* table, int Ns, T* kout, double* part,
                         hipStream_t s);
 
+    // Rough-surface synthesis on an n x n plan (rows/cols in and out = n):
+    // out (n^2, row-major) = c2r of the random-phase self-affine spectrum,
+    // scaled by `scale` (see RbGen). Register-blocked sizes, T = double.
+    void generate_surface(double hurst, double k_low, double k_high, double scale,
+                          unsigned long long seed, T* out, hipStream_t s);
+
     int row_out_blocks() const;   // blocks of rows_inv (partial count)
     int batch_capacity() const { return batch_; }
     // Symbol::table layout the kernels of this plan read: the register-

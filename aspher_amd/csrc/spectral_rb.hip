@@ -445,6 +445,79 @@ void launch_permute(const T* kin, int kp, int nh, T* kout, hipStream_t s) {
     k_rb_permute_table<T, LOGN><<<grid, 256, 0, s>>>(kin, kp, nh, kout);
     ASPHER_LAUNCH_CHECK();
 }
+
+// ── rough-surface generator: spectrum synthesis + inverse column pass ─────
+__device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned long long idx) {
+    // SplitMix64 output number idx of a generator seeded with `seed`
+    unsigned long long z = seed + (idx + 1ull) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(512)
+k_rb_gen_cols(cx<double>* __restrict__ BR, int pitch, double hurst, double k_low,
+              double k_high, double scale, unsigned long long seed,
+              const cx<double>* __restrict__ tw, int W, int remap) {
+    using Sh = rb::Shape<LOGN>;
+    constexpr int n = Sh::n, TPC = Sh::threads, nh = n / 2 + 1;
+    constexpr int LL = rb::lds_len(n);
+    constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tile = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int c = threadIdx.x / TPC, th = threadIdx.x - c * TPC;
+    const int kx = tile * W + c;
+    const bool live = kx < nh;
+    cx<double>* lds = reinterpret_cast<cx<double>*>(smem) + c * LL;
+    const double two_pi = 6.283185307179586476925286766559;
+    cx<double> r[16];
+#pragma unroll
+    for (int q = 0; q < Gl; ++q)
+#pragma unroll
+        for (int v = 0; v < Nl; ++v) {
+            const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+            cx<double> h = {0.0, 0.0};
+            if (live) {
+                const int mx = kx < n / 2 ? kx : kx - n;   // fftfreq order
+                const int my = ky < n / 2 ? ky : ky - n;
+                const double mag = sqrt((double)mx * mx + (double)my * my);
+                if (mag >= k_low && mag <= k_high && mag > 0.0) {
+                    const double A = 0.5 * scale * pow(mag, -(1.0 + hurst));
+                    const int kxn = (n - kx) % n, kyn = (n - ky) % n;
+                    const double p1 = two_pi * hash_uniform(seed, (unsigned long long)ky * n + kx);
+                    const double p2 = two_pi * hash_uniform(seed, (unsigned long long)kyn * n + kxn);
+                    double s1, c1, s2, c2;
+                    sincos(p1, &s1, &c1);
+                    sincos(p2, &s2, &c2);
+                    h = {A * (c1 + c2), A * (s1 - s2)};
+                }
+            }
+            r[q * Nl + v] = h;
+        }
+    rb::inverse<double, LOGN>(r, lds, th, tw, 1);
+    if (live) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int p = Sh::pos(0, th, v);
+            BR[(size_t)p * pitch + kx] = r[v];
+        }
+    }
+}
+
+template <int LOGN>
+void launch_gen(const RbGen& a, hipStream_t s) {
+    constexpr int n = 1 << LOGN;
+    const int block = a.W * (n / 16);
+    const int grid = (n / 2 + 1 + a.W - 1) / a.W;
+    const std::size_t lds = sizeof(cx<double>) * (std::size_t)a.W * rb::lds_len(n);
+    const void* fn = (const void*)k_rb_gen_cols<LOGN>;
+    allow_lds_rb(fn, lds);
+    k_rb_gen_cols<LOGN><<<grid, block, lds, s>>>(a.BR, a.pitch, a.hurst, a.k_low, a.k_high,
+                                                 a.scale, a.seed, a.tw, a.W, a.remap);
+    ASPHER_LAUNCH_CHECK();
+}
 } // namespace
 
 template <typename T>
@@ -494,6 +567,21 @@ void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s) {
         case 12: launch_rows_inv<T, 12>(a, s); break;
         case 13: launch_rows_inv<T, 13>(a, s); break;
         default: throw std::invalid_argument("row transform length out of range");
+    }
+}
+
+void rb_gen_cols(int logN, const RbGen& a, hipStream_t s) {
+    switch (logN) {
+        case 5: launch_gen<5>(a, s); break;
+        case 6: launch_gen<6>(a, s); break;
+        case 7: launch_gen<7>(a, s); break;
+        case 8: launch_gen<8>(a, s); break;
+        case 9: launch_gen<9>(a, s); break;
+        case 10: launch_gen<10>(a, s); break;
+        case 11: launch_gen<11>(a, s); break;
+        case 12: launch_gen<12>(a, s); break;
+        case 13: launch_gen<13>(a, s); break;
+        default: throw std::invalid_argument("surface size out of range");
     }
 }
 

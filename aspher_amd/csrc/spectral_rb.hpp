@@ -73,6 +73,22 @@ struct RbRowsInv {
     RbBatch bt;
 };
 
+// Self-affine random-phase spectrum, column pass of the surface generator:
+// BR[ky][kx] (ky < n, kx <= n/2) = inverse column DFT of
+//   H(kx, ky) = scale/2 * A(m) * (exp(i phi(k)) + exp(-i phi(-k))),
+// A(m) = |m|^-(1+hurst) for k_low <= |m| <= k_high (else 0), m the signed
+// frequency of k, phi(k) = 2 pi u(seed, ky*n + kx) from a SplitMix64 hash.
+struct RbGen {
+    cx<double>* BR = nullptr;
+    int pitch = 0;
+    double hurst = 0.8, k_low = 4.0, k_high = 64.0, scale = 1.0;
+    unsigned long long seed = 0;
+    const cx<double>* tw = nullptr;
+    int W = 1;
+    int remap = 1;
+};
+void rb_gen_cols(int logN, const RbGen& a, hipStream_t s);
+
 template <typename T> void rb_rows_fwd(int logL, const RbRowsFwd<T>& a, hipStream_t s);
 template <typename T> void rb_cols(int logN, const RbCols<T>& a, hipStream_t s);
 template <typename T> void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s);

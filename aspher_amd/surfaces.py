@@ -33,6 +33,32 @@ def rough_gap(Ns: int, **kw) -> np.ndarray:
     return np.ascontiguousarray(-rough_surface(Ns, **kw).ravel())
 
 
+def rough_surface_gpu(Ns: int, H: float = 0.8, k_low: float = 4, k_high: float = 64,
+                      rms: float = 0.002, seed: int = 0) -> np.ndarray:
+    """The same recipe generated on the GPU (libaspher_hip), with the phase of
+    frequency (kx, ky) from a SplitMix64 hash of (seed, ky*Ns + kx) instead
+    of numpy's generator. (Ns, Ns) float64 on the host."""
+    import ctypes as C
+    from ._capi import check, dptr, lib
+    out = np.empty(Ns * Ns, dtype=np.float64)
+    check(lib().aspher_rough_surface(int(Ns), float(H), float(k_low), float(k_high), float(rms),
+                                     C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), dptr(out)))
+    return out.reshape(Ns, Ns)
+
+
+def rough_surface_to_device(dev_array, Ns: int, H: float = 0.8, k_low: float = 4,
+                            k_high: float = 64, rms: float = 0.002, seed: int = 0) -> None:
+    """Generate straight into a device buffer of Ns^2 float64 (a DeviceArray
+    or a raw device address): the surface never leaves HBM."""
+    import ctypes as C
+    from ._capi import check, lib
+    ptr = dev_array if isinstance(dev_array, int) else dev_array.ptr
+    check(lib().aspher_rough_surface_device(int(Ns), float(H), float(k_low), float(k_high),
+                                            float(rms),
+                                            C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF),
+                                            C.c_void_p(ptr)))
+
+
 def westergaard_gap(Ns: int, L: float = 1.0, amplitude: float = 0.01) -> np.ndarray:
     """Single sinusoid g0 = D (1 - cos 2 pi x / L) along x (Westergaard 1939)."""
     x = np.arange(Ns) / Ns

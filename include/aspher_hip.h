@@ -293,6 +293,19 @@ int aspher_solve_ensemble_device(int n, const int* backend, const int* grid_size
                                  aspher_result** results, aspher_ensemble_stats* stats,
                                  double* instance_seconds);
 
+/* Self-affine Gaussian rough surface generated on the device. The
+ * reference's benchmarks draw their surfaces from the third-party `rfgen`
+ * package (pyproject.toml:15), outside the library; this replaces it with
+ * the published recipe: amplitude |m|^-(1+hurst) for integer wavenumber
+ * magnitudes k_low <= |m| <= k_high, uniform random phase, real part of the
+ * inverse 2-D DFT, mean removed, scaled to `rms`. The phase of frequency
+ * (kx, ky) is 2 pi times SplitMix64 output number ky*Ns + kx of a generator
+ * seeded with `seed` (53-bit uniform). Ns^2 doubles, row-major. */
+int aspher_rough_surface(int grid_size, double hurst, double k_low, double k_high, double rms,
+                         uint64_t seed, double* out);
+int aspher_rough_surface_device(int grid_size, double hurst, double k_low, double k_high,
+                                double rms, uint64_t seed, double* out_dev);
+
 /* Device buffers for callers of the *_device entry points that have no
  * allocator of their own (tests, benchmarks). No reference counterpart: the
  * reference is host-only. */

@@ -29,6 +29,9 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
   kkt_gradient             central finite differences through the exact
                            support-enumeration solver and solve_contact
   circulant_matrix         equals the operator's matvec on random vectors
+  splitmix64               the published known-answer vector (seed 1234567)
+  rough_surface_hash       band limit, mean/rms, Hermitian output, equality
+                           with the numpy-phase recipe on the same phases
 """
 from __future__ import annotations
 
@@ -922,3 +925,48 @@ def circulant_matrix(matvec, Ns):
         for jx in range(Ns):
             S[:, jy * Ns + jx] = np.roll(np.roll(c0, jy, axis=0), jx, axis=1).ravel()
     return S
+
+
+# ── rough-surface generator (the benchmark recipe; rfgen is third-party) ───
+#
+# The reference's benchmarks draw surfaces from `rfgen` (pyproject.toml:15,
+# bench_backend_precision_study.py:37-59), which is not part of the
+# reference tree and not installed. Recipe: amplitude |m|^-(1+H) for integer
+# wavenumber magnitude in [k_low, k_high] (fftfreq order), uniform random
+# phase, real part of ifft2, mean removed, scaled to rms. The device
+# generator draws the phase of frequency (kx, ky) as SplitMix64 output number
+# ky*Ns + kx (Steele, Lea, Flood, OOPSLA 2014) so that it can be restated here.
+
+_SM_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, idx):
+    """Output number idx (0-based) of a SplitMix64 generator seeded with seed."""
+    z = np.uint64(seed) + (np.asarray(idx, dtype=np.uint64) + np.uint64(1)) * _SM_GOLDEN
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def hash_uniform(seed, idx):
+    """53-bit uniform in [0, 1) from splitmix64."""
+    return (splitmix64(seed, idx) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def rough_surface_from_phases(Ns, H, k_low, k_high, rms, phase):
+    """phase: (Ns, Ns) in [0, 1), indexed [ky, kx]."""
+    k = np.fft.fftfreq(Ns, 1.0 / Ns)
+    kk = np.hypot(k[:, None], k[None, :])
+    kk[0, 0] = 1.0
+    amp = np.where((kk >= k_low) & (kk <= k_high), kk ** (-(1.0 + H)), 0.0)
+    amp[0, 0] = 0.0
+    h = np.real(np.fft.ifft2(amp * np.exp(2j * np.pi * phase)))
+    h -= h.mean()
+    h *= rms / h.std()
+    return h
+
+
+def rough_surface_hash(Ns, H=0.8, k_low=4.0, k_high=64.0, rms=0.002, seed=0):
+    """The device generator's surface (phases from splitmix64)."""
+    idx = np.arange(Ns * Ns, dtype=np.uint64).reshape(Ns, Ns)
+    return rough_surface_from_phases(Ns, H, k_low, k_high, rms, hash_uniform(seed, idx))
