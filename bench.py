@@ -75,13 +75,16 @@ def run_solve(hc, Ns, gptr, precision, reuse, float_floor):
     return hc.solve_nested_device(Ns, gptr, 0.005, tol=1e-6, **kw)
 
 
-def timed(fn, steps, warmup):
+def timed(fn, steps, warmup, each=None):
     r = None
     for _ in range(warmup):
         r = fn()
     t0 = time.perf_counter()
     for _ in range(steps):
+        t1 = time.perf_counter()
         r = fn()
+        if each is not None:
+            each.append(time.perf_counter() - t1)
     return (time.perf_counter() - t0) / steps, r
 
 
@@ -181,6 +184,8 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the 64-instance batched workload")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ledger", default=None,
+                    help="append JSONL records of the timed solves (reference ledger format)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -241,12 +246,13 @@ def main():
     g = DeviceArray.from_numpy(gap, np.float64)
     ff = args.float_floor
     log("[bench] mixed solves (build included)")
+    w_mixed, w_fp64 = [], []
     t_mixed, r_mixed = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", False, ff),
-                             args.steps, args.warmup)
+                             args.steps, args.warmup, w_mixed)
     log("[bench]   %.3f s  %s" % (t_mixed, r_mixed))
     log("[bench] fp64 solves (build included)")
     t_fp64, r_fp64 = timed(lambda: run_solve(hc, Ns, g.ptr, "double", False, ff),
-                           args.steps, args.warmup)
+                           args.steps, args.warmup, w_fp64)
     log("[bench]   %.3f s  %s" % (t_fp64, r_fp64))
     log("[bench] solve-only (operators cached)")
     t_mixed_so, _ = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff),
@@ -283,6 +289,20 @@ def main():
                     matvec_gbs=round(m32["gbs"], 1),
                     matvec_frac=round(m32["gbs"] / HBM_PEAK_GBS, 4),
                     matvec_ms=round(m32["ms"], 4))
+
+    if args.ledger:
+        from aspher_amd import ledger
+        prov = ledger.provenance()
+        wp = dict(kind="selfaffine", Hurst=0.8, rms=0.002, k_low=4, k_high=64, seed=0,
+                  p_bar=0.005)
+        for variant, prec, res, walls in (("fft-mixed", "float_then_double", r_mixed, w_mixed),
+                                          ("fft-f64", "double", r_fp64, w_fp64)):
+            sargs = dict(grid_size=Ns, p_nominal=0.005, coarsest=64, backend="fft",
+                         precision=prec, tol=1e-6, coarse_tol=1e-4, max_iter=20000,
+                         precond=True, light_result=False)
+            ledger.append(args.ledger, ledger.solve_record("rough-H0.8-rms0.002", Ns, variant,
+                                                           wp, gap, sargs, res, walls), prov)
+        log("[bench] ledger records appended to %s" % args.ledger)
 
     ens = None
     if not args.no_ensemble:

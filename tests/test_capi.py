@@ -84,3 +84,40 @@ def test_product_package_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 txt = open(os.path.join(dirpath, f), errors="replace").read()
                 assert not pat.search(txt), os.path.join(dirpath, f)
+
+
+def test_ledger_record_keys(tmp_path):
+    """Ledger records carry the reference harness's keys (bench/harness.py:514-560)."""
+    import json
+    from types import SimpleNamespace
+    import numpy as np
+    from aspher_amd import ledger
+    res = SimpleNamespace(
+        contact_area=0.1, memory=dict(cg_state=1, best_iterate=2, contact_mask=3,
+                                      output_arrays=4, peak=10),
+        timings=dict(total=1.0, matvec=0.5, precond=0.1, build=0.2, coarse=0.1,
+                     verification=0.0, candidate=0.0, output=0.0),
+        status="converged", status_reason="", converged=True, iterations=10,
+        matvec_count=21, verification_matvec_count=0, precond_count=10,
+        precond_dropped=False, identification_steps=0, requested_tol=1e-6,
+        effective_tol=1e-6, validation_scope="solve_operator", operator_error_kind="unavailable",
+        fw_error=1e-7, penetration_error=1e-7, load_error=0.0, pk_error=1e-6,
+        mean_pressure=0.005, active_rounds=0, active_fallback=False,
+        stage_stats=[dict(name="finest:64")], pressure=np.ones(16))
+    rec = ledger.solve_record("w", 4, "v", dict(p_bar=0.005), np.zeros(16), dict(grid_size=4),
+                              res, [1.0, 0.5, 0.7])
+    expect = {"workload", "Ns", "N", "variant", "surface_sha", "workload_params", "p_bar",
+              "occupancy", "solver_args", "wall_cold_s", "wall_warm_s", "wall_all_s",
+              "surface_gen_s", "solver_memory", "timings", "status", "status_reason",
+              "converged", "iterations", "matvec_count", "verification_matvec_count",
+              "precond_count", "precond_dropped", "identification_steps", "requested_tol",
+              "effective_tol", "validation_scope", "operator_error_kind", "fw_error",
+              "penetration_error", "load_error", "pk_error", "contact_area", "mean_pressure",
+              "active_rounds", "active_fallback", "stage_stats", "pressure_sha", "run_status",
+              "timestamp"}
+    assert set(rec) == expect
+    assert rec["wall_warm_s"] == 0.7 and rec["pressure_sha"] == ledger.digest(np.ones(16))
+    path = tmp_path / "l.jsonl"
+    ledger.append(str(path), rec, prov={"revision": "x"})
+    back = json.loads(path.read_text().strip())
+    assert back["provenance"] == {"revision": "x"} and back["N"] == 16
