@@ -482,7 +482,7 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
         finalize<Real>(part.p, 0, 2, kS2, dl1, scal.p, STEP_LINE1, s, ev);
         V::line2(Ntot, R, T_, bmask.p, scal.p, part.p, 0, s, ev);
         finalize<Real>(part.p, 0, 1, kS1, dl2, scal.p, STEP_LINE2, s, ev);
-        V::update(Ntot, P, T_, G, scal.p, part.p, 0, s, ev);
+        V::update(Ntot, P, P, T_, G, scal.p, part.p, 0, s, ev);
         finalize<Real>(part.p, 0, 2, ou, du, scal.p, STEP_UPDATE, s, ev);
         V::rescale(Ntot, P, scal.p, s, ev);
     };

@@ -294,15 +294,17 @@ k_line2(std::int64_t N, const T* __restrict__ r, const T* __restrict__ t,
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_update(std::int64_t N, T* __restrict__ p, const T* __restrict__ t,
+k_update(std::int64_t N, const T* pin, T* pout, const T* __restrict__ t,
          const T* __restrict__ g, const double* __restrict__ scal,
          double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
     ENS_SETUP(N);
-    p += o_; t += o_; g += o_;
+    pin += o_; pout += o_; t += o_; g += o_;
     scal = ENS_SCAL(scal);
     double* pr = ENS_PART(part);
     if (scal[S_OK] == 0.0) {
+        // no usable curvature: the pressure is unchanged
+        if (pout != pin) ENS_LOOP(i) pout[i] = pin[i];
         if (threadIdx.x == 0) {
             pr[0] = 0.0;
             pr[1] = 0.0;
@@ -313,7 +315,7 @@ k_update(std::int64_t N, T* __restrict__ p, const T* __restrict__ t,
     const T tau_ov = static_cast<T>(scal[S_TAUOV]);
     double changed = 0.0, total = 0.0;
     ENS_LOOP(i) {
-        T pi = p[i];
+        T pi = pin[i];
         const bool was = pi > T(0);
         if (was) {
             const T q = pi - tau * t[i];
@@ -322,7 +324,7 @@ k_update(std::int64_t N, T* __restrict__ p, const T* __restrict__ t,
         const T gi = g[i];
         if (pi == T(0) && gi < T(0)) pi = pi - tau_ov * gi;
         if ((pi > T(0)) != was) changed = 1.0;
-        p[i] = pi;
+        pout[i] = pi;
         total += (double)pi;
     }
     putr<RMAX>(changed, red, pr, 0);
@@ -1014,9 +1016,10 @@ void VecOps<T>::line2(std::int64_t N, const T* r, const T* t, const std::uint8_t
     LE(k_line2<T>, N, r, t, mask, scal, part, ens);
 }
 template <typename T>
-void VecOps<T>::update(std::int64_t N, T* p, const T* t, const T* g, const double* scal,
-                       double* part, int nblk, hipStream_t s, const Ens& ens) {
-    LE(k_update<T>, N, p, t, g, scal, part, ens);
+void VecOps<T>::update(std::int64_t N, const T* pin, T* pout, const T* t, const T* g,
+                       const double* scal, double* part, int nblk, hipStream_t s,
+                       const Ens& ens) {
+    LE(k_update<T>, N, pin, pout, t, g, scal, part, ens);
 }
 template <typename T>
 void VecOps<T>::rescale(std::int64_t N, T* p, const double* scal, hipStream_t s,

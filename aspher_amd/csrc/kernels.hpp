@@ -73,8 +73,9 @@ template <typename T> struct VecOps {
     static void line2(std::int64_t N, const T* r, const T* t, const std::uint8_t* mask,
                       const double* scal, double* part, int nblk, hipStream_t s,
                       const Ens& ens = Ens{});
-    // projected update + overlap step (predicated on S_OK): partials {changed, total}
-    static void update(std::int64_t N, T* p, const T* t, const T* g,
+    // projected update + overlap step pout = update(pin) (predicated on S_OK,
+    // else pout = pin; pin == pout allowed): partials {changed, total}
+    static void update(std::int64_t N, const T* pin, T* pout, const T* t, const T* g,
                        const double* scal, double* part, int nblk, hipStream_t s,
                        const Ens& ens = Ens{});
     // load rescale (predicated on S_OK)

@@ -108,8 +108,11 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     }
     const Real gshift = subtract ? static_cast<Real>(datum) : Real(0);
 
-    // device state: p, t, u, g, g_prev, z, r (+ best p, mask, centred gap)
-    DevBuf<Real> bg0s(N), bp(N), bt(N), bu(N), bg(N), bgp(N), bz(N), br(N), bbest;
+    // device state: p, t, u, g, g_prev, z, r (+ mask, centred gap). The
+    // pressure rotates through two buffers (three with keep_best): the
+    // projected update writes out of place, so marking the best iterate is
+    // an index, not a copy of p.
+    DevBuf<Real> bg0s(N), bp(N), bp2(N), bt(N), bu(N), bg(N), bgp(N), bz(N), br(N), bbest;
     DevBuf<std::uint8_t> bmask(N);
     Real *P = bp.p, *T_ = bt.p, *U = bu.p, *G = bg.p, *GP = bgp.p, *Z = bz.p, *R = br.p;
     Real* g0s = bg0s.p;
@@ -137,6 +140,14 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         set_scal(scal, 6, idx, val, s);
     }
     if (opt.keep_best) bbest.alloc(N);
+    Real* pb[3] = {bp.p, bp2.p, bbest.p};
+    const int nbuf = opt.keep_best ? 3 : 2;
+    int cur = 0, best_idx = -1;
+    auto pick_out = [&]() -> int {
+        for (int i = 0; i < nbuf; ++i)
+            if (i != cur && i != best_idx) return i;
+        throw std::logic_error("solve_contact: no free pressure buffer");
+    };
 
     long long mv = 0, pcount = 0;
     EventTimer tm_mv, tm_pc;
@@ -210,7 +221,12 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             sync_read();
             const double f_try = hs[S_R0], gd = hs[S_R1];
             if (std::isfinite(f_try) && gd < 0.0 && f_try <= f_cur + 1e-4 * gd) {
-                std::swap(P, Z);
+                {
+                    const int o = pick_out();   // Z is reused; keep p in the rotation
+                    V::copy(N, Z, pb[o], s);
+                    cur = o;
+                    P = pb[o];
+                }
                 std::swap(U, R);
                 u_fresh = true;
                 set_delta_restart();
@@ -264,9 +280,12 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         finalize<Real>(part, nblk, 2, kS2, dl1, scal, STEP_LINE1, s);
         V::line2(N, R, T_, mask, scal, part, nblk, s);
         finalize<Real>(part, nblk, 1, kS1, dl2, scal, STEP_LINE2, s);
-        V::update(N, P, T_, G, scal, part, nblk, s);
+        const int o = pick_out();
+        V::update(N, P, pb[o], T_, G, scal, part, nblk, s);
         finalize<Real>(part, nblk, 2, ou, du, scal, STEP_UPDATE, s);
-        V::rescale(N, P, scal, s);
+        V::rescale(N, pb[o], scal, s);
+        cur = o;
+        P = pb[o];
     };
 
     Status status = Status::max_iterations;
@@ -315,7 +334,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         const double merit = std::max(d.fw, d.pen);
         if (opt.keep_best && merit < best_merit) {
             best_merit = merit;
-            V::copy(N, P, bbest.p, s);
+            best_idx = cur;
             have_best = true;
         }
         if (merit < seen_merit * (1.0 - 1e-4)) {
@@ -352,12 +371,12 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     if (status != Status::converged && opt.keep_best && have_best) {
         const double merit = std::max(d.fw, d.pen);
         if (!(merit <= best_merit)) {
-            V::copy(N, bbest.p, P, s);
+            cur = best_idx;
+            P = pb[cur];
             u_fresh = false;
             res.returned_best = true;
         }
     }
-    bbest.reset();
     if (!u_fresh) apply(P, U);
     evaluate();
     sync_read();
@@ -413,7 +432,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     res.precond_count = pcount;
     res.precond_dropped = precond_dropped;
     const long long rb = sizeof(Real);
-    res.mem_cg_state = 7 * N * rb;
+    res.mem_cg_state = 8 * N * rb;
     res.mem_best = opt.keep_best ? N * rb : 0;
     res.mem_mask = N;
     res.mem_output = (rb == 8 ? 0 : N * 8) + (opt.light ? 0 : 2 * N * 8);
