@@ -30,6 +30,12 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
                            support-enumeration solver and solve_contact
   circulant_matrix         equals the operator's matvec on random vectors
   splitmix64               the published known-answer vector (seed 1234567)
+  cerruti_uxx / _uxy       Gauss-Legendre quadrature (tests/test_cerruti.cpp:53-78),
+                           nu = 0 -> love_uz, parity, closed-form self term
+  CerrutiTables            accessor identities, dense block symmetric + PD,
+                           continuum symbol eigenvalues
+  TangentialOperator       equals the dense block at Ns in {8, 12, 16}
+  TangentialPreconditioner symmetry, mask, mean removal
   rough_surface_hash       band limit, mean/rms, Hermitian output, equality
                            with the numpy-phase recipe on the same phases
 """
@@ -77,6 +83,12 @@ def lib():
         L.oracle_love_many.argtypes = [ctypes.c_int64, pd, pd, pd, pd, pd]
         L.oracle_love_table.restype = None
         L.oracle_love_table.argtypes = [ctypes.c_int, d, d, pd]
+        L.oracle_cerruti_uxx.restype = d
+        L.oracle_cerruti_uxx.argtypes = [d, d, d, d, d]
+        L.oracle_cerruti_uxy.restype = d
+        L.oracle_cerruti_uxy.argtypes = [d, d, d, d]
+        L.oracle_cerruti_many.argtypes = [ctypes.c_int64, pd, pd, pd, pd, d, pd, pd]
+        L.oracle_cerruti_tables.argtypes = [ctypes.c_int, d, d, d, pd, pd]
         _lib = L
     return _lib
 
@@ -970,3 +982,145 @@ def rough_surface_hash(Ns, H=0.8, k_low=4.0, k_high=64.0, rms=0.002, seed=0):
     """The device generator's surface (phases from splitmix64)."""
     idx = np.arange(Ns * Ns, dtype=np.uint64).reshape(Ns, Ns)
     return rough_surface_from_phases(Ns, H, k_low, k_high, rms, hash_uniform(seed, idx))
+
+
+# ── tangential (Cerruti) operator: src/cerruti_kernel.cpp,
+#    src/tangential_operator.cpp:16-157, src/fourier_precond.cpp:234-359 ──────
+
+def cerruti_uxx(x, y, a, b, nu):
+    """Vectorised cerruti_uxx bracket (src/cerruti_kernel.cpp:85-95)."""
+    x, y, a, b = (np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64),
+                                                        np.broadcast(x, y, a, b).shape)).ravel()
+                  for v in (x, y, a, b))
+    uxx, uxy = np.empty_like(x), np.empty_like(x)
+    lib().oracle_cerruti_many(x.size, _ptr(x), _ptr(y), _ptr(a), _ptr(b), float(nu),
+                              _ptr(uxx), _ptr(uxy))
+    return uxx
+
+
+def cerruti_uxy(x, y, a, b):
+    """Vectorised cerruti_uxy bracket (src/cerruti_kernel.cpp:97-123)."""
+    x, y, a, b = (np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64),
+                                                        np.broadcast(x, y, a, b).shape)).ravel()
+                  for v in (x, y, a, b))
+    uxx, uxy = np.empty_like(x), np.empty_like(x)
+    lib().oracle_cerruti_many(x.size, _ptr(x), _ptr(y), _ptr(a), _ptr(b), 0.0,
+                              _ptr(uxx), _ptr(uxy))
+    return uxy
+
+
+class CerrutiTables:
+    """CerrutiKernel (include/cerruti_kernel.hpp:34-72): xx and xy tables and
+    the signed-offset accessors."""
+
+    def __init__(self, Ns, L=1.0, E_star=1.0, nu=0.3):
+        self.Ns, self.L, self.E, self.nu = Ns, L, E_star, nu
+        self.xx = np.empty((Ns, Ns))
+        self.xy = np.empty((Ns, Ns))
+        lib().oracle_cerruti_tables(int(Ns), float(L), float(E_star), float(nu),
+                                    _ptr(self.xx), _ptr(self.xy))
+
+    def xx_off(self, dx, dy):
+        dx, dy = np.abs(dx), np.abs(dy)
+        ok = (dx < self.Ns) & (dy < self.Ns)
+        return np.where(ok, self.xx[np.minimum(dy, self.Ns - 1), np.minimum(dx, self.Ns - 1)], 0.0)
+
+    def yy_off(self, dx, dy):
+        return self.xx_off(dy, dx)
+
+    def xy_off(self, dx, dy):
+        dx, dy = np.asarray(dx), np.asarray(dy)
+        ax, ay = np.abs(dx), np.abs(dy)
+        ok = (ax < self.Ns) & (ay < self.Ns) & (dx != 0) & (dy != 0)
+        v = self.xy[np.minimum(ay, self.Ns - 1), np.minimum(ax, self.Ns - 1)]
+        return np.where(ok, np.where((dx > 0) == (dy > 0), v, -v), 0.0)
+
+    def dense(self):
+        """[[XX, XY], [XY, YY]] (src/cerruti_kernel.cpp:129-144)."""
+        Ns = self.Ns
+        i = np.arange(Ns * Ns)
+        dix = (i % Ns)[:, None] - (i % Ns)[None, :]
+        diy = (i // Ns)[:, None] - (i // Ns)[None, :]
+        xx, xy, yy = self.xx_off(dix, diy), self.xy_off(dix, diy), self.yy_off(dix, diy)
+        return np.block([[xx, xy], [xy, yy]])
+
+    def symbol(self, kx, ky):
+        """Continuum 2x2 symbol (src/cerruti_kernel.cpp:146-161)."""
+        k2 = kx * kx + ky * ky
+        c = 2.0 / (self.E * (1.0 - self.nu) * np.sqrt(k2))
+        return np.array([[c * (1 - self.nu * kx * kx / k2), -c * self.nu * kx * ky / k2],
+                         [-c * self.nu * kx * ky / k2, c * (1 - self.nu * ky * ky / k2)]])
+
+
+class TangentialOperator:
+    """TangentialFFTOperator (src/tangential_operator.cpp:16-157): the three
+    tables embedded in wrap-around order on the (2Ns)^2 grid, real spectra
+    (checked), u = [Kxx Kxy; Kxy Kyy] q per mode; q, u stacked [x; y]."""
+
+    def __init__(self, Ns, L=1.0, E_star=1.0, nu=0.3):
+        self.Ns = Ns
+        self.k = CerrutiTables(Ns, L, E_star, nu)
+        M = 2 * Ns
+        d = np.arange(M)
+        d = np.where(d < Ns, d, d - M)
+        DX, DY = np.meshgrid(d, d, indexing="xy")      # [iy, ix]
+        zero = (np.arange(M)[None, :] == Ns) | (np.arange(M)[:, None] == Ns)
+        self.K = {}
+        for name, f in (("xx", self.k.xx_off), ("yy", self.k.yy_off), ("xy", self.k.xy_off)):
+            E = np.where(zero, 0.0, f(DX, DY))
+            C = _fwd(E)
+            if np.abs(C.imag).max() > 1e-10 * np.abs(C.real).max():
+                raise RuntimeError("tangential spectrum not real")
+            self.K[name] = C.real / (M * M)
+
+    def matvec(self, q):
+        Ns, M = self.Ns, 2 * self.Ns
+        N = Ns * Ns
+        q = np.asarray(q, dtype=np.float64)
+        G1 = np.zeros((M, M))
+        G2 = np.zeros((M, M))
+        G1[:Ns, :Ns] = q[:N].reshape(Ns, Ns)
+        G2[:Ns, :Ns] = q[N:].reshape(Ns, Ns)
+        A, B = _fwd(G1), _fwd(G2)
+        U1 = self.K["xx"] * A + self.K["xy"] * B
+        U2 = self.K["xy"] * A + self.K["yy"] * B
+        u1, u2 = _inv(U1, M), _inv(U2, M)
+        return np.concatenate([u1[:Ns, :Ns].ravel(), u2[:Ns, :Ns].ravel()])
+
+
+class TangentialPreconditioner:
+    """TangentialFourierPreconditioner (src/fourier_precond.cpp:234-359):
+    masked scatter, per-mode 2x2 multiply by |k|/Ns^2 (I + gamma k k^T/|k|^2),
+    gamma = nu/(1-nu), cross term zeroed on self-conjugate Nyquist modes,
+    masked gather and optional per-component mask-mean removal."""
+
+    def __init__(self, Ns, nu):
+        self.Ns = Ns
+        g = nu / (1.0 - nu)
+        norm = 1.0 / (Ns * Ns)
+        ky = np.arange(Ns)
+        fy = np.where(ky < Ns // 2, ky, ky - Ns).astype(np.float64)[:, None]
+        fx = np.arange(Ns // 2 + 1, dtype=np.float64)[None, :]
+        k2 = fx * fx + fy * fy
+        kk = np.sqrt(k2)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.wxx = np.where(k2 > 0, norm * kk * (1 + g * fx * fx / k2), 0.0)
+            self.wyy = np.where(k2 > 0, norm * kk * (1 + g * fy * fy / k2), 0.0)
+            nyq = (fx == Ns // 2) | (fy == Ns // 2) | (np.arange(Ns)[:, None] == Ns // 2)
+            self.wxy = np.where((k2 > 0) & ~nyq, norm * kk * g * fx * fy / k2, 0.0)
+
+    def apply(self, g, mask, remove_mean=False):
+        Ns = self.Ns
+        N = Ns * Ns
+        m = np.asarray(mask).astype(bool).ravel()
+        g = np.asarray(g, dtype=np.float64)
+        A = _fwd(np.where(m, g[:N], 0.0).reshape(Ns, Ns))
+        B = _fwd(np.where(m, g[N:], 0.0).reshape(Ns, Ns))
+        Z1 = _inv(self.wxx * A + self.wxy * B, Ns).ravel()
+        Z2 = _inv(self.wxy * A + self.wyy * B, Ns).ravel()
+        z1 = np.where(m, Z1, 0.0)
+        z2 = np.where(m, Z2, 0.0)
+        if remove_mean and m.any():
+            z1[m] -= Z1[m].sum() / m.sum()
+            z2[m] -= Z2[m].sum() / m.sum()
+        return np.concatenate([z1, z2])

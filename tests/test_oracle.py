@@ -501,3 +501,108 @@ def test_rough_surface_hash_recipe():
     np.testing.assert_allclose(a, b, rtol=0, atol=1e-15)
     # a different seed is a different surface
     assert np.abs(O.rough_surface_hash(Ns, H, kl, kh, rms, seed=4) - h).max() > 0.1 * rms
+
+
+# ── tangential (Cerruti) operator ───────────────────────────────────────────
+
+def _gauss_legendre(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return x, w
+
+
+def test_cerruti_brackets_vs_quadrature():
+    """tests/test_cerruti.cpp:53-78: 64-point tensor Gauss-Legendre, 1e-9."""
+    h = 1.0 / 32
+    a, nu = 0.5 * h, 0.3
+    xs, ws = _gauss_legendre(64)
+    XI, ETA = np.meshgrid(a * xs, a * xs, indexing="ij")
+    W = np.outer(ws, ws) * a * a
+    for ox, oy in ((1, 0), (0, 2), (1, 1), (2, 1), (5, 3), (9, 4)):
+        x, y = ox * h, oy * h
+        s = np.hypot(x - XI, y - ETA)
+        qxx = (W * ((1 - nu) / s + nu * (x - XI) ** 2 / s ** 3)).sum()
+        qxy = (W * ((x - XI) * (y - ETA) / s ** 3)).sum()
+        assert abs(O.cerruti_uxx(x, y, a, a, nu)[0] / qxx - 1) < 1e-9
+        if ox and oy:
+            assert abs(O.cerruti_uxy(x, y, a, a)[0] / qxy - 1) < 1e-9
+
+
+def test_cerruti_identities():
+    """tests/test_cerruti.cpp:80-101."""
+    a, b, x, y, nu = 0.011, 0.017, 0.05, -0.03, 0.31
+    assert abs(O.cerruti_uxx(x, y, a, b, 0.0)[0] / O.love_uz(x, y, a, b)[0] - 1) < 1e-13
+    assert O.cerruti_uxx(x, y, a, b, nu)[0] == O.cerruti_uxx(-x, y, a, b, nu)[0]
+    assert O.cerruti_uxx(x, y, a, b, nu)[0] == O.cerruti_uxx(x, -y, a, b, nu)[0]
+    assert O.cerruti_uxy(-x, y, a, b)[0] == -O.cerruti_uxy(x, y, a, b)[0]
+    assert O.cerruti_uxy(x, -y, a, b)[0] == -O.cerruti_uxy(x, y, a, b)[0]
+    self_t = (2 - nu) * 4 * a * math.log(1 + math.sqrt(2))
+    assert abs(O.cerruti_uxx(0.0, 0.0, a, a, nu)[0] / self_t - 1) < 1e-14
+    assert O.cerruti_uxy(0.0, 0.0, a, a)[0] == 0.0
+
+
+def test_cerruti_tables_dense_block():
+    """tests/test_cerruti.cpp:103-143: accessors, symmetric PD block."""
+    Ns, L, E, nu = 16, 1.0, 1.7, 0.3
+    C = O.CerrutiTables(Ns, L, E, nu)
+    a = 0.5 * L / Ns
+    self_t = (2 - nu) * 4 * a * math.log(1 + math.sqrt(2)) / (math.pi * E * (1 - nu))
+    assert abs(C.xx_off(0, 0) / self_t - 1) < 1e-14 and C.xy_off(0, 0) == 0
+    assert C.yy_off(3, 1) == C.xx_off(1, 3) and C.xx_off(-3, 1) == C.xx_off(3, 1)
+    assert C.xy_off(2, 5) == C.xy_off(-2, -5) and C.xy_off(-2, 5) == -C.xy_off(2, 5)
+    assert C.xy_off(0, 4) == 0 and C.xy_off(4, 0) == 0 and C.xx_off(Ns, 0) == 0
+    M = O.CerrutiTables(8, L, E, nu).dense()
+    assert np.array_equal(M, M.T)
+    np.linalg.cholesky(M)
+
+
+def test_cerruti_symbol():
+    """tests/test_cerruti.cpp:145-190: eigenvalues of the continuum symbol and
+    the lattice DFT of the tables against it."""
+    C = O.CerrutiTables(128, 1.0, 1.7, 0.3)
+    kx, ky = 7.3, -2.1
+    kk = math.hypot(kx, ky)
+    S = C.symbol(kx, ky)
+    el, et = np.array([kx, ky]) / kk, np.array([-ky, kx]) / kk
+    assert abs(el @ S @ el / (2 / (1.7 * kk)) - 1) < 1e-14
+    assert abs(et @ S @ et / (2 / (1.7 * 0.7 * kk)) - 1) < 1e-14
+    h = 1.0 / 128
+    d = np.arange(-127, 128)
+    DX, DY = np.meshgrid(d, d, indexing="xy")
+    for mx, my in ((8, 0), (0, 8), (6, 6), (12, 5)):
+        qx, qy = 2 * math.pi * mx, 2 * math.pi * my
+        c = np.cos(qx * DX * h + qy * DY * h)
+        S = C.symbol(qx, qy)
+        assert abs((C.xx_off(DX, DY) * c).sum() / S[0, 0] - 1) < 0.18
+        assert abs((C.yy_off(DX, DY) * c).sum() / S[1, 1] - 1) < 0.18
+        if mx and my:
+            # odd table: the sine-sine part carries the cross symbol
+            sxy = -(C.xy_off(DX, DY) * np.sin(qx * DX * h) * np.sin(qy * DY * h)).sum()
+            assert abs(sxy / S[0, 1] - 1) < 0.10
+
+
+@pytest.mark.parametrize("Ns", [8, 12, 16])
+def test_tangential_fft_equals_dense(Ns):
+    """tests/test_tangential.cpp:27-40 (rel < 1e-13)."""
+    T = O.TangentialOperator(Ns, 1.0, 1.3, 0.29)
+    M = O.CerrutiTables(Ns, 1.0, 1.3, 0.29).dense()
+    q = np.random.default_rng(Ns).standard_normal(2 * Ns * Ns)
+    u = T.matvec(q)
+    ref = M @ q
+    assert np.linalg.norm(u - ref) / np.linalg.norm(ref) < 1e-13
+
+
+def test_tangential_preconditioner_symmetric():
+    Ns = 16
+    P = O.TangentialPreconditioner(Ns, 0.3)
+    rng = np.random.default_rng(1)
+    mask = rng.random(Ns * Ns) < 0.6
+    a, b = rng.standard_normal(2 * Ns * Ns), rng.standard_normal(2 * Ns * Ns)
+    m2 = np.concatenate([mask, mask])
+    a[~m2] = 0
+    b[~m2] = 0
+    za, zb = P.apply(a, mask), P.apply(b, mask)
+    assert abs(za @ b - zb @ a) < 1e-12 * np.abs(za @ b)
+    assert np.all(za[~m2] == 0)
+    zm = P.apply(a, mask, remove_mean=True)
+    N = Ns * Ns
+    assert abs(zm[:N][mask].sum()) < 1e-12 and abs(zm[N:][mask].sum()) < 1e-12
