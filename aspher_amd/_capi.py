@@ -30,6 +30,9 @@ EXPORTS = (
     "aspher_solver_adjoint", "aspher_solver_adjoint_device", "aspher_copy_device",
     "aspher_ensemble_params_default", "aspher_solve_ensemble", "aspher_solve_ensemble_device",
     "aspher_rough_surface", "aspher_rough_surface_device",
+    "aspher_tangential_create", "aspher_tangential_destroy", "aspher_tangential_matvec",
+    "aspher_tangential_matvec_device", "aspher_tangential_precond_apply",
+    "aspher_cerruti_points", "aspher_cerruti_tables",
 )
 
 
@@ -157,6 +160,13 @@ def _declare(L):
         "aspher_copy_device": (i, [vp, vp, C.c_int64]),
         "aspher_ensemble_params_default": (None, [C.POINTER(EnsembleParams)]),
         "aspher_rough_surface": (i, [i, d, d, d, d, C.c_uint64, dp]),
+        "aspher_tangential_create": (i, [i, d, d, d, C.POINTER(vp)]),
+        "aspher_tangential_destroy": (None, [vp]),
+        "aspher_tangential_matvec": (i, [vp, dp, dp]),
+        "aspher_tangential_matvec_device": (i, [vp, vp, vp, i]),
+        "aspher_tangential_precond_apply": (i, [i, d, i, dp, C.POINTER(C.c_uint8), i, dp]),
+        "aspher_cerruti_points": (i, [i, dp, dp, dp, dp, d, dp, dp]),
+        "aspher_cerruti_tables": (i, [i, d, d, d, dp, dp]),
         "aspher_rough_surface_device": (i, [i, d, d, d, d, C.c_uint64, vp]),
         "aspher_solve_ensemble": (i, [i, ip, ip, dp, dp, dp, C.POINTER(dp),
                                       C.POINTER(EnsembleParams), C.POINTER(vp),

@@ -26,6 +26,12 @@ struct aspher_result {
     Result r;
 };
 
+struct aspher_tangential {
+    std::unique_ptr<Context> ctx;
+    std::unique_ptr<TangentialOperator> op;
+    int Ns = 0;
+};
+
 namespace {
 
 thread_local std::string g_err;
@@ -790,6 +796,124 @@ int aspher_solve_ensemble_device(int n, const int* backend, const int* grid_size
     return guard([&] {
         ensemble_run(n, backend, grid_size, domain_size, E_star, p_nominal, gaps_dev, true,
                      params, results, stats, instance_seconds);
+    });
+}
+
+
+int aspher_tangential_create(int grid_size, double domain_size, double E_star, double nu,
+                             aspher_tangential** out) {
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null output handle");
+        auto t = std::make_unique<aspher_tangential>();
+        t->ctx = std::make_unique<Context>();
+        t->op = std::make_unique<TangentialOperator>(grid_size, domain_size, E_star, nu);
+        t->op->build(t->ctx->stream);
+        t->Ns = grid_size;
+        *out = t.release();
+    });
+}
+
+void aspher_tangential_destroy(aspher_tangential* t) {
+    if (!t) return;
+    try {
+        t->op.reset();
+        delete t;
+    } catch (...) {
+    }
+}
+
+int aspher_tangential_matvec(aspher_tangential* t, const double* q, double* u) {
+    return guard([&] {
+        if (!t || !q || !u) throw std::invalid_argument("null pointer");
+        const std::size_t n2 = (std::size_t)2 * t->Ns * t->Ns;
+        DevBuf<double> x(n2);
+        hipStream_t st = t->ctx->stream;
+        ASPHER_HIP(hipMemcpyAsync(x.p, q, sizeof(double) * n2, hipMemcpyHostToDevice, st));
+        t->op->apply(x.p, x.p, st);
+        ASPHER_HIP(hipMemcpyAsync(u, x.p, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_tangential_matvec_device(aspher_tangential* t, const void* q, void* u, int precision) {
+    return guard([&] {
+        if (!t || !q || !u) throw std::invalid_argument("null pointer");
+        ASPHER_HIP(hipDeviceSynchronize());
+        hipStream_t st = t->ctx->stream;
+        if (precision == 64)
+            t->op->apply(static_cast<const double*>(q), static_cast<double*>(u), st);
+        else if (precision == 32)
+            t->op->apply(static_cast<const float*>(q), static_cast<float*>(u), st);
+        else
+            throw std::invalid_argument("precision must be 64 or 32");
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_tangential_precond_apply(int grid_size, double nu, int precision, const double* g,
+                                    const uint8_t* mask, int remove_mean, double* z) {
+    return guard([&] {
+        if (!g || !mask || !z) throw std::invalid_argument("null pointer");
+        if (precision != 64 && precision != 32)
+            throw std::invalid_argument("precision must be 64 or 32");
+        TangentialPrecond pc(grid_size, nu);
+        Context ctx;
+        hipStream_t st = ctx.stream;
+        const std::size_t N = (std::size_t)grid_size * grid_size;
+        DevBuf<std::uint8_t> m(N);
+        DevBuf<double> gd(2 * N), zd(2 * N);
+        ASPHER_HIP(hipMemcpyAsync(m.p, mask, N, hipMemcpyHostToDevice, st));
+        ASPHER_HIP(hipMemcpyAsync(gd.p, g, sizeof(double) * 2 * N, hipMemcpyHostToDevice, st));
+        if (precision == 64) {
+            pc.apply<double>(gd.p, m.p, remove_mean != 0, zd.p, st);
+        } else {
+            DevBuf<float> gf(2 * N), zf(2 * N);
+            VecOps<float>::from_double((std::int64_t)(2 * N), gd.p, 0.0, gf.p, st);
+            pc.apply<float>(gf.p, m.p, remove_mean != 0, zf.p, st);
+            VecOps<float>::to_double((std::int64_t)(2 * N), zf.p, zd.p, st);
+            ASPHER_HIP(hipStreamSynchronize(st));
+        }
+        ASPHER_HIP(hipMemcpyAsync(z, zd.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_cerruti_points(int n, const double* x, const double* y, const double* a,
+                          const double* b, double nu, double* uxx, double* uxy) {
+    return guard([&] {
+        if (n < 0 || !x || !y || !a || !b || !uxx || !uxy)
+            throw std::invalid_argument("bad arguments");
+        if (n == 0) return;
+        Context ctx;
+        hipStream_t st = ctx.stream;
+        DevBuf<double> d(6 * (std::size_t)n);
+        const double* src[4] = {x, y, a, b};
+        for (int k = 0; k < 4; ++k)
+            ASPHER_HIP(hipMemcpyAsync(d.p + (std::size_t)k * n, src[k], sizeof(double) * n,
+                                      hipMemcpyHostToDevice, st));
+        cerruti_points(n, d.p, d.p + n, d.p + 2 * (std::size_t)n, d.p + 3 * (std::size_t)n, nu,
+                       d.p + 4 * (std::size_t)n, d.p + 5 * (std::size_t)n, st);
+        ASPHER_HIP(hipMemcpyAsync(uxx, d.p + 4 * (std::size_t)n, sizeof(double) * n,
+                                  hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipMemcpyAsync(uxy, d.p + 5 * (std::size_t)n, sizeof(double) * n,
+                                  hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int aspher_cerruti_tables(int grid_size, double domain_size, double E_star, double nu,
+                          double* xx, double* xy) {
+    return guard([&] {
+        if (grid_size <= 0 || !xx || !xy) throw std::invalid_argument("bad arguments");
+        if (!(nu >= 0.0 && nu < 1.0)) throw std::invalid_argument("nu must be in [0, 1)");
+        Context ctx;
+        hipStream_t st = ctx.stream;
+        const std::size_t N = (std::size_t)grid_size * grid_size;
+        DevBuf<double> a(N), b(N);
+        cerruti_tables(grid_size, domain_size, E_star, nu, a.p, b.p, st);
+        ASPHER_HIP(hipMemcpyAsync(xx, a.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipMemcpyAsync(xy, b.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
     });
 }
 

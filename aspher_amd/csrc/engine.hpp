@@ -166,6 +166,52 @@ private:
     DevBuf<float> tmp32_;
 };
 
+// Tangential (Cerruti) 2x2 block operator on stacked [x; y] components
+// (src/tangential_operator.cpp:16-157): FP64 tables and spectra, FP32 copies
+// on demand. Power-of-two grids 16 .. 2048.
+class TangentialOperator {
+public:
+    TangentialOperator(int Ns, double L, double E_star, double nu);
+    ~TangentialOperator();
+    TangentialOperator(const TangentialOperator&) = delete;
+    TangentialOperator& operator=(const TangentialOperator&) = delete;
+    void build(hipStream_t s);
+    void build_single(hipStream_t s);
+    // u = C q, 2 Ns^2 values each; q and u may alias
+    void apply(const double* q, double* u, hipStream_t s);
+    void apply(const float* q, float* u, hipStream_t s);
+    int Ns() const { return Ns_; }
+    double build_seconds = 0.0;
+
+private:
+    template <typename T> void apply_t(const T* q, T* u, hipStream_t s);
+    int Ns_;
+    double L_, E_, nu_;
+    bool built_ = false;
+    DevBuf<double> k64_[3];      // unique (Ns+1)^2 spectra: xx, yy, xy
+    DevBuf<double> perm64_[3];   // register-order tables of the FP64 plan
+    DevBuf<float> perm32_[3];
+    std::unique_ptr<SpectralPlan<double>> plan64_;
+    std::unique_ptr<SpectralPlan<float>> plan32_;
+};
+
+// Tangential Fourier preconditioner (src/fourier_precond.cpp:234-359): masked
+// scatter of both components, per-mode |k|/Ns^2 (I + gamma k k^T/|k|^2),
+// gamma = nu/(1-nu), masked gather, optional per-component mask-mean removal.
+class TangentialPrecond {
+public:
+    TangentialPrecond(int Ns, double nu);
+    template <typename T>
+    void apply(const T* g, const std::uint8_t* mask, bool remove_mean, T* z, hipStream_t s);
+
+private:
+    int Ns_;
+    double nu_;
+    DevBuf<double> scal_, part_;
+    std::unique_ptr<SpectralPlan<double>> plan64_;
+    std::unique_ptr<SpectralPlan<float>> plan32_;
+};
+
 // Preconditioner: none, 13-tap stencil (src/stencil_precond.cpp) or full-grid
 // Fourier |k| (src/fourier_precond.cpp).
 class Precond {

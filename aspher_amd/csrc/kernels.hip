@@ -793,6 +793,63 @@ __device__ double d_love_uz(double x, double y, double a, double b) {
            d_term(ym, xm, xp, -da);
 }
 
+// ── Cerruti (tangential) cell integrals (src/cerruti_kernel.cpp:1-127) ─────
+// uxx = (1-nu) love_uz + nu ylog (ylog = integral of (x-xi)^2/rho^3, same
+// far-field switch); uxy = integral of (x-xi)(y-eta)/rho^3 in the
+// rationalised, cancellation-free product form, odd in x and in y.
+__device__ double d_cerruti_ylog(double xa, double ya, double a, double b) {
+    const double mmax = (a > b) ? a : b;
+    const double r2 = xa * xa + ya * ya;
+    const double sw = 1500.0 * mmax;
+    if (r2 > sw * sw) {
+        const double r = sqrt(r2);
+        const double r3 = r2 * r, r5 = r3 * r2, r7 = r5 * r2;
+        const double A = 4.0 * a * b;
+        const double x2 = xa * xa, y2 = ya * ya;
+        return A * (x2 / r3 + (a * a / 6.0) * (2.0 / r3 - 15.0 * x2 / r5 + 15.0 * x2 * x2 / r7) +
+                    (b * b / 6.0) * (-3.0 * x2 / r5 + 15.0 * x2 * y2 / r7));
+    }
+    const double k = xa + a, l = xa - a, m = ya + b, n = ya - b;
+    const double da = 2.0 * a;
+    return d_term(m, k, l, da) + d_term(n, l, k, -da);
+}
+
+__device__ double d_cerruti_uxx(double x, double y, double a, double b, double nu) {
+    const double xa = fabs(x), ya = fabs(y);
+    return (1.0 - nu) * d_love_uz(xa, ya, a, b) + nu * d_cerruti_ylog(xa, ya, a, b);
+}
+
+__device__ double d_cerruti_uxy(double x, double y, double a, double b) {
+    const double xa = fabs(x), ya = fabs(y);
+    const double k = xa + a, l = xa - a, m = ya + b, n = ya - b;
+    const double Rkm = hypot(k, m), Rkn = hypot(k, n), Rlm = hypot(l, m), Rln = hypot(l, n);
+    const double num = 16.0 * a * b * xa * ya * (1.0 / (Rkn + Rln) + 1.0 / (Rkm + Rlm));
+    const double den = (Rlm + Rln) * (Rkn + Rkm);
+    const double v = den > 0.0 ? num / den : 0.0;
+    return ((x >= 0.0) ? 1.0 : -1.0) * ((y >= 0.0) ? 1.0 : -1.0) * v;
+}
+
+__global__ void __launch_bounds__(256)
+k_cerruti_tables(int Ns, double h, double nu, double pref, double prefnu,
+                 double* __restrict__ xx, double* __restrict__ xy) {
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    const double a = 0.5 * h;
+    GRID_LOOP(i, N) {
+        const int dy = (int)(i / Ns), dx = (int)(i - (std::int64_t)dy * Ns);
+        xx[i] = pref * d_cerruti_uxx(dx * h, dy * h, a, a, nu);
+        xy[i] = prefnu * d_cerruti_uxy(dx * h, dy * h, a, a);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_cerruti_points(int n, const double* x, const double* y, const double* a, const double* b,
+                 double nu, double* uxx, double* uxy) {
+    GRID_LOOP(i, n) {
+        uxx[i] = d_cerruti_uxx(x[i], y[i], a[i], b[i], nu);
+        uxy[i] = d_cerruti_uxy(x[i], y[i], a[i], b[i]);
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_love_table(int Ns, double h, double scale, double* __restrict__ table) {
     const std::int64_t N = (std::int64_t)Ns * Ns;
@@ -1188,6 +1245,22 @@ void set_scal(double* scal, int n, const int* idx, const double* val, hipStream_
         sp.val[i] = val[i];
     }
     k_set_scal<<<1, 64, 0, s>>>(scal, sp);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void cerruti_tables(int Ns, double L, double E_star, double nu, double* xx, double* xy,
+                    hipStream_t s) {
+    const double pi = 3.14159265358979323846;
+    const double pref = 1.0 / (pi * E_star * (1.0 - nu));
+    const std::int64_t N = (std::int64_t)Ns * Ns;
+    k_cerruti_tables<<<vec_blocks(N), kVecThreads, 0, s>>>(Ns, L / Ns, nu, pref, nu * pref, xx,
+                                                           xy);
+    ASPHER_LAUNCH_CHECK();
+}
+
+void cerruti_points(int n, const double* x, const double* y, const double* a, const double* b,
+                    double nu, double* uxx, double* uxy, hipStream_t s) {
+    k_cerruti_points<<<vec_blocks(n), kVecThreads, 0, s>>>(n, x, y, a, b, nu, uxx, uxy);
     ASPHER_LAUNCH_CHECK();
 }
 

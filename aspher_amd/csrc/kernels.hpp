@@ -163,6 +163,14 @@ void love_table(int Ns, double L, double E_star, double* table, hipStream_t s);
 void love_points(int n, const double* x, const double* y, const double* a,
                  const double* b, double* out, hipStream_t s);
 
+// Cerruti tables (src/cerruti_kernel.cpp:117-127): xx[dy*Ns+dx], xy[dy*Ns+dx]
+// for dx, dy in [0, Ns), prefactors 1/(pi E* (1-nu)) and nu/(pi E* (1-nu)).
+void cerruti_tables(int Ns, double L, double E_star, double nu, double* xx, double* xy,
+                    hipStream_t s);
+// Cerruti brackets at arbitrary points (tests of the device kernel)
+void cerruti_points(int n, const double* x, const double* y, const double* a, const double* b,
+                    double nu, double* uxx, double* uxy, hipStream_t s);
+
 // 2x2 block-average restriction (Ns -> Ns/2) and injection prolongation.
 void restrict2(int Ns, const double* f, double* out, hipStream_t s);
 void prolong2(int Nc, const double* pc, double* out, hipStream_t s);

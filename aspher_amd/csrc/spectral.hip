@@ -464,9 +464,10 @@ void SpectralPlan<T>::generate_surface(double hurst, double k_low, double k_high
 }
 
 template <typename T>
-void SpectralPlan<T>::permute_table(const T* kunique, int kp, T* kout, hipStream_t s) const {
+void SpectralPlan<T>::permute_table(const T* kunique, int kp, T* kout, hipStream_t s,
+                                    bool odd) const {
     if (!rb_) throw std::logic_error("permute_table: generic plans read the unique table");
-    rb_permute_table<T>(logn_, kunique, kp, nh, kout, s);
+    rb_permute_table<T>(logn_, kunique, kp, nh, kout, s, odd);
 }
 
 template <typename T>
@@ -669,7 +670,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
 
 template <typename T>
 void SpectralPlan<T>::build_spectrum(const double* table, int Ns, T* kout, double* part,
-                                     hipStream_t s) {
+                                     hipStream_t s, int embed) {
     if (rows_in != n) throw std::invalid_argument("build_spectrum: plan must read all rows");
     if (rb_) {
         RbRowsFwd<T> f;
@@ -679,7 +680,7 @@ void SpectralPlan<T>::build_spectrum(const double* table, int Ns, T* kout, doubl
         f.pitchT = pitchT_;
         f.B = row_batch_;
         f.tw = tw64_->dev;
-        f.mode = 2;
+        f.mode = embed;
         f.table = table;
         f.Ns_embed = Ns;
         f.remap = xcd_remap_ ? 1 : 0;
@@ -700,8 +701,79 @@ void SpectralPlan<T>::build_spectrum(const double* table, int Ns, T* kout, doubl
         rb_cols<T>(logn_, c, s);
         return;
     }
+    if (embed != 2) throw std::invalid_argument("build_spectrum: table kinds need a power-of-two plan");
     rows_fwd(nullptr, 0, RowIn::embed, nullptr, nullptr, 0, table, Ns, n, n, s);
     cols(Symbol::build, nullptr, nh, 0.0, kout, part, n, 0, s);
+}
+
+template <typename T>
+void SpectralPlan<T>::convolve2(const T* x, int x_pitch, std::int64_t x_stride, T* y,
+                                int y_pitch, std::int64_t y_stride, int sym, const T* kxx,
+                                const T* kyy, const T* kxy, int kp, double gamma, double norm,
+                                hipStream_t s, RowIn in_mode, RowOut out_mode,
+                                const std::uint8_t* mask, const double* scal, int gmean_idx,
+                                double* part, std::int64_t pstride) {
+    if (!rb_ || batch_ < 2 || n > 4096)
+        throw std::invalid_argument("convolve2: needs a power-of-two plan of batch >= 2, n <= 4096");
+    RbBatch bt;
+    bt.count = 2;
+    bt.x_stride = x_stride;
+    bt.y_stride = y_stride;
+    bt.m_stride = 0;
+    bt.at_stride = at_stride_;
+    bt.br_stride = br_stride_;
+    bt.pstride = pstride;
+    RbRowsFwd<T> f;
+    f.x = x;
+    f.x_pitch = x_pitch;
+    f.ncols = cols_in;
+    f.nrows = rows_in;
+    f.AT = AT_;
+    f.pitchT = pitchT_;
+    f.B = row_batch_;
+    f.tw = tw64_->dev;
+    f.mode = in_mode == RowIn::masked ? 1 : 0;
+    f.mask = mask;
+    f.scal = scal;
+    f.gmean_idx = gmean_idx;
+    f.remap = xcd_remap_ ? 1 : 0;
+    f.bt = bt;
+    RbCols2<T> c;
+    c.AT = AT_;
+    c.pitchT = pitchT_;
+    c.at_stride = at_stride_;
+    c.BR = A_;
+    c.pitch = pitch;
+    c.br_stride = br_stride_;
+    c.ncols = nh;
+    c.rows_in = rows_in;
+    c.rows_out = rows_out;
+    c.tw = f.tw;
+    c.sym = sym;
+    c.kxx = kxx;
+    c.kyy = kyy;
+    c.kxy = kxy;
+    c.kp = kp;
+    c.gamma = gamma;
+    c.norm = norm;
+    c.remap = f.remap;
+    RbRowsInv<T> r;
+    r.BR = A_;
+    r.pitch = pitch;
+    r.nrows = rows_out;
+    r.B = row_batch_;
+    r.tw = f.tw;
+    r.y = y;
+    r.y_pitch = y_pitch;
+    r.ncols = cols_out;
+    r.mode = out_mode == RowOut::masked ? 1 : 0;
+    r.mask = mask;
+    r.part = part;
+    r.remap = f.remap;
+    r.bt = bt;
+    rb_rows_fwd<T>(ilog2(n / 2), f, s);
+    rb_cols2<T>(logn_, c, s);
+    rb_rows_inv<T>(ilog2(n / 2), r, s);
 }
 
 template class Twiddles<float>;

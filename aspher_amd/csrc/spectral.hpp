@@ -97,8 +97,23 @@ public:
     // Forward-only pass used to build the free-space kernel spectrum from
     // the Love table (T = double): writes Re/n^2 into kout[(kx)(n/2+1)+ky]
     // for ky in [0, n/2] and per-block max |Im|, max |Re| into part.
+    // embed: 2 even table, 3 even transposed (yy from xx), 4 odd-odd (xy)
     void build_spectrum(const double* table, int Ns, T* kout, double* part,
-                        hipStream_t s);
+                        hipStream_t s, int embed = 2);
+
+    // Two-component (tangential) convolution on a plan of batch >= 2: the x
+    // and y components at x and x + x_stride (outputs y, y + y_stride) are
+    // transformed, mixed per mode by a real symmetric 2x2 symbol
+    // (sym 5: tables in the register order, sym 6: the tangential
+    // preconditioner symbol with `gamma`, `norm`) and transformed back.
+    // Masked modes share one mask for both components; with RowOut::masked
+    // the per-block sums of component c go to part + c*pstride.
+    void convolve2(const T* x, int x_pitch, std::int64_t x_stride, T* y, int y_pitch,
+                   std::int64_t y_stride, int sym, const T* kxx, const T* kyy, const T* kxy,
+                   int kp, double gamma, double norm, hipStream_t s,
+                   RowIn in_mode = RowIn::plain, RowOut out_mode = RowOut::plain,
+                   const std::uint8_t* mask = nullptr, const double* scal = nullptr,
+                   int gmean_idx = 0, double* part = nullptr, std::int64_t pstride = 0);
 
     // Rough-surface synthesis on an n x n plan (rows/cols in and out = n):
     // out (n^2, row-major) = c2r of the random-phase self-affine spectrum,
@@ -113,7 +128,8 @@ public:
     // of n values, pitch n); the generic path the unique (n/2+1)-wide table.
     bool table_permuted() const { return rb_; }
     std::size_t table_elems() const { return (std::size_t)nh * n; }
-    void permute_table(const T* kunique, int kp, T* kout, hipStream_t s) const;
+    void permute_table(const T* kunique, int kp, T* kout, hipStream_t s,
+                       bool odd = false) const;
     SpectralTuning tuning() const { return {row_batch_, col_width_, xcd_remap_}; }
     int col_blocks() const;       // blocks of cols (partial count, build)
     std::size_t workspace_bytes() const;

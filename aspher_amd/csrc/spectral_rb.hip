@@ -48,7 +48,7 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
         const int li = blockIdx.y, gi = bt.first + li;
         if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
         x += li * bt.x_stride;
-        if (mask) mask += li * bt.x_stride;
+        if (mask) mask += li * (bt.m_stride >= 0 ? bt.m_stride : bt.x_stride);
         AT += li * bt.at_stride;
         if (scal) scal += (std::size_t)gi * bt.sstride;
     }
@@ -62,12 +62,23 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
 
     auto load = [&](int col) -> T {
         if (row >= nrows || col >= ncols) return T(0);
-        if constexpr (IN == 2) {
+        if constexpr (IN >= 2) {
+            // wrap-around embedding of an offset table (index d or n + d;
+            // row/column Ns zero): 2 even, 3 even transposed, 4 odd-odd
             const int Ns = Ns_embed;
             if (row == Ns || col == Ns) return T(0);
-            const int ady = row < Ns ? row : n - row;
-            const int adx = col < Ns ? col : n - col;
-            return static_cast<T>(table[(size_t)ady * Ns + adx]);
+            const int dy = row < Ns ? row : row - n;
+            const int dx = col < Ns ? col : col - n;
+            const int ady = dy < 0 ? -dy : dy, adx = dx < 0 ? -dx : dx;
+            if constexpr (IN == 2) {
+                return static_cast<T>(table[(size_t)ady * Ns + adx]);
+            } else if constexpr (IN == 3) {
+                return static_cast<T>(table[(size_t)adx * Ns + ady]);
+            } else {
+                if (dx == 0 || dy == 0) return T(0);
+                const T v = static_cast<T>(table[(size_t)ady * Ns + adx]);
+                return ((dx > 0) == (dy > 0)) ? v : T(-v);
+            }
         } else {
             const size_t i = (size_t)row * x_pitch + col;
             if constexpr (IN == 1) return mask[i] ? T(x[i] - gmean) : T(0);
@@ -256,7 +267,7 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
         if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
         BR += li * bt.br_stride;
         y += li * bt.y_stride;
-        if (mask) mask += li * bt.y_stride;
+        if (mask) mask += li * (bt.m_stride >= 0 ? bt.m_stride : bt.y_stride);
         if (part) part += (std::size_t)gi * bt.pstride;
     }
     constexpr int L = Sh::n, TPR = Sh::threads;
@@ -355,10 +366,18 @@ void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
         fn = (const void*)k_rb_rows_fwd<T, LOGL, 1, false>;
         allow_lds_rb(fn, lds);
         k_rb_rows_fwd<T, LOGL, 1, false><<<grid, block, lds, s>>>(RF_ARGS);
-    } else {
+    } else if (a.mode == 2) {
         fn = (const void*)k_rb_rows_fwd<T, LOGL, 2, false>;
         allow_lds_rb(fn, lds);
         k_rb_rows_fwd<T, LOGL, 2, false><<<grid, block, lds, s>>>(RF_ARGS);
+    } else if (a.mode == 3) {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 3, false>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_fwd<T, LOGL, 3, false><<<grid, block, lds, s>>>(RF_ARGS);
+    } else {
+        fn = (const void*)k_rb_rows_fwd<T, LOGL, 4, false>;
+        allow_lds_rb(fn, lds);
+        k_rb_rows_fwd<T, LOGL, 4, false><<<grid, block, lds, s>>>(RF_ARGS);
     }
 #undef RF_ARGS
     ASPHER_LAUNCH_CHECK();
@@ -419,13 +438,122 @@ void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
 }
 
 
+// ── two-component (tangential) column pass ───────────────────────────────
+// One workgroup per column kx: threads [0, TPC) transform the x component
+// (plan instance 0), threads [TPC, 2 TPC) the y component (instance 1). After
+// the forward transforms both halves swap their spectra through LDS (the two
+// components hold the same digit-reversed positions), mix them by the real
+// symmetric 2x2 symbol and run their inverse transforms.
+template <typename T, int LOGN, int SYM>
+__global__ void __launch_bounds__(512)
+k_rb_cols2(const cx<T>* __restrict__ AT, int pitchT, std::int64_t at_stride,
+           cx<T>* __restrict__ BR, int pitch, std::int64_t br_stride, int ncols, int rows_in,
+           int rows_out, const cx<double>* __restrict__ tw, const T* __restrict__ kxx,
+           const T* __restrict__ kyy, const T* __restrict__ kxy, int kp, double gamma,
+           double norm, int remap) {
+    using Sh = rb::Shape<LOGN>;
+    constexpr int n = Sh::n, TPC = Sh::threads;
+    constexpr int LL = rb::lds_len(n);
+    constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
+    constexpr bool ZP = (SYM == 5);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int kx = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int c = threadIdx.x / TPC, th = threadIdx.x - c * TPC;
+    const bool live = kx < ncols;
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem) + c * LL;
+    const cx<T>* ldo = reinterpret_cast<cx<T>*>(smem) + (1 - c) * LL;
+
+    cx<T> r[16];
+    const cx<T>* col = AT + c * at_stride + (size_t)(live ? kx : 0) * pitchT;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int p = Sh::pos(0, th, v);
+        if (ZP && v >= 8) r[v] = {T(0), T(0)};
+        else r[v] = (live && p < rows_in) ? col[p] : cx<T>{T(0), T(0)};
+    }
+    rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lds[rb::lds_ix(16 * th + i)] = r[i];
+    __syncthreads();
+    cx<T> o[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) o[i] = ldo[rb::lds_ix(16 * th + i)];
+    if constexpr (SYM == 5) {
+        const std::size_t at = (size_t)(live ? kx : 0) * kp + 16 * th;
+        T kd[16], kc[16];
+        load16((c == 0 ? kxx : kyy) + at, kd);
+        load16(kxy + at, kc);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const T a = live ? kd[i] : T(0), b = live ? kc[i] : T(0);
+            r[i] = {a * r[i].re + b * o[i].re, a * r[i].im + b * o[i].im};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Gl; ++q)
+#pragma unroll
+            for (int v = 0; v < Nl; ++v) {
+                const int i = q * Nl + v;
+                const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+                const double fx = (double)kx;
+                const double fy = (double)(ky < (n >> 1) ? ky : ky - n);
+                const double k2 = fx * fx + fy * fy;
+                double wd = 0.0, wc = 0.0;
+                if (live && k2 > 0.0) {
+                    const double kk = sqrt(k2);
+                    const double f = c == 0 ? fx : fy;
+                    wd = norm * kk * (1.0 + gamma * f * f / k2);
+                    wc = (kx == (n >> 1) || ky == (n >> 1)) ? 0.0
+                                                            : norm * kk * gamma * fx * fy / k2;
+                }
+                const T a = static_cast<T>(wd), b = static_cast<T>(wc);
+                r[i] = {a * r[i].re + b * o[i].re, a * r[i].im + b * o[i].im};
+            }
+    }
+    rb::inverse<T, LOGN, ZP>(r, lds, th, tw, 1);
+    if (live) {
+        cx<T>* out = BR + c * br_stride;
+#pragma unroll
+        for (int v = 0; v < (ZP ? 8 : 16); ++v) {
+            const int p = Sh::pos(0, th, v);
+            if (p < rows_out) out[(size_t)p * pitch + kx] = r[v];
+        }
+    }
+}
+
+template <typename T, int LOGN>
+void launch_cols2(const RbCols2<T>& a, hipStream_t s) {
+    constexpr int n = 1 << LOGN;
+    static_assert(2 * (n / 16) <= 512, "two-component columns up to n = 4096");
+    const int block = 2 * (n / 16);
+    const int grid = a.ncols;
+    const std::size_t lds = sizeof(cx<T>) * 2 * (std::size_t)rb::lds_len(n);
+    if (a.sym == 5) {
+        if (a.rows_in > n / 2 || a.rows_out > n / 2)
+            throw std::invalid_argument("tangential columns need rows <= n/2");
+        const void* fn = (const void*)k_rb_cols2<T, LOGN, 5>;
+        allow_lds_rb(fn, lds);
+        k_rb_cols2<T, LOGN, 5><<<grid, block, lds, s>>>(
+            a.AT, a.pitchT, a.at_stride, a.BR, a.pitch, a.br_stride, a.ncols, a.rows_in,
+            a.rows_out, a.tw, a.kxx, a.kyy, a.kxy, a.kp, a.gamma, a.norm, a.remap);
+    } else {
+        const void* fn = (const void*)k_rb_cols2<T, LOGN, 6>;
+        allow_lds_rb(fn, lds);
+        k_rb_cols2<T, LOGN, 6><<<grid, block, lds, s>>>(
+            a.AT, a.pitchT, a.at_stride, a.BR, a.pitch, a.br_stride, a.ncols, a.rows_in,
+            a.rows_out, a.tw, a.kxx, a.kyy, a.kxy, a.kp, a.gamma, a.norm, a.remap);
+    }
+    ASPHER_LAUNCH_CHECK();
+}
+
 // ── free-space kernel spectrum in the column kernel's register order ──────
 // kout[kx*n + pos] = K(kx, ky) with ky = kidx(pos) the frequency that the
 // forward column transform leaves at digit-reversed position pos, folded by
 // evenness to the (n/2+1)-wide unique table kin[kx*kp + min(ky, n-ky)].
 template <typename T, int LOGN>
 __global__ void __launch_bounds__(256)
-k_rb_permute_table(const T* __restrict__ kin, int kp, int nh, T* __restrict__ kout) {
+k_rb_permute_table(const T* __restrict__ kin, int kp, int nh, T* __restrict__ kout, bool odd) {
     using Sh = rb::Shape<LOGN>;
     constexpr int n = Sh::n;
     const std::int64_t total = (std::int64_t)nh * n;
@@ -434,15 +562,16 @@ k_rb_permute_table(const T* __restrict__ kin, int kp, int nh, T* __restrict__ ko
         const int kx = (int)(i / n), pos = (int)(i - (std::int64_t)kx * n);
         const int ky = Sh::kidx(pos);
         const int kyp = (ky <= (n >> 1)) ? ky : n - ky;
-        kout[i] = kin[(std::size_t)kx * kp + kyp];
+        const T v = kin[(std::size_t)kx * kp + kyp];
+        kout[i] = (odd && ky > (n >> 1)) ? T(-v) : v;
     }
 }
 
 template <typename T, int LOGN>
-void launch_permute(const T* kin, int kp, int nh, T* kout, hipStream_t s) {
+void launch_permute(const T* kin, int kp, int nh, T* kout, hipStream_t s, bool odd) {
     const std::int64_t total = (std::int64_t)nh * (1 << LOGN);
     const int grid = (int)std::min<std::int64_t>((total + 255) / 256, 65536);
-    k_rb_permute_table<T, LOGN><<<grid, 256, 0, s>>>(kin, kp, nh, kout);
+    k_rb_permute_table<T, LOGN><<<grid, 256, 0, s>>>(kin, kp, nh, kout, odd);
     ASPHER_LAUNCH_CHECK();
 }
 
@@ -570,6 +699,23 @@ void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s) {
     }
 }
 
+template <typename T>
+void rb_cols2(int logN, const RbCols2<T>& a, hipStream_t s) {
+    switch (logN) {
+        case 5: launch_cols2<T, 5>(a, s); break;
+        case 6: launch_cols2<T, 6>(a, s); break;
+        case 7: launch_cols2<T, 7>(a, s); break;
+        case 8: launch_cols2<T, 8>(a, s); break;
+        case 9: launch_cols2<T, 9>(a, s); break;
+        case 10: launch_cols2<T, 10>(a, s); break;
+        case 11: launch_cols2<T, 11>(a, s); break;
+        case 12: launch_cols2<T, 12>(a, s); break;
+        default: throw std::invalid_argument("two-component transform length out of range");
+    }
+}
+template void rb_cols2<float>(int, const RbCols2<float>&, hipStream_t);
+template void rb_cols2<double>(int, const RbCols2<double>&, hipStream_t);
+
 void rb_gen_cols(int logN, const RbGen& a, hipStream_t s) {
     switch (logN) {
         case 5: launch_gen<5>(a, s); break;
@@ -586,22 +732,24 @@ void rb_gen_cols(int logN, const RbGen& a, hipStream_t s) {
 }
 
 template <typename T>
-void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s) {
+void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s,
+                      bool odd) {
     switch (logN) {
-        case 5: launch_permute<T, 5>(kin, kp, nh, kout, s); break;
-        case 6: launch_permute<T, 6>(kin, kp, nh, kout, s); break;
-        case 7: launch_permute<T, 7>(kin, kp, nh, kout, s); break;
-        case 8: launch_permute<T, 8>(kin, kp, nh, kout, s); break;
-        case 9: launch_permute<T, 9>(kin, kp, nh, kout, s); break;
-        case 10: launch_permute<T, 10>(kin, kp, nh, kout, s); break;
-        case 11: launch_permute<T, 11>(kin, kp, nh, kout, s); break;
-        case 12: launch_permute<T, 12>(kin, kp, nh, kout, s); break;
-        case 13: launch_permute<T, 13>(kin, kp, nh, kout, s); break;
+        case 5: launch_permute<T, 5>(kin, kp, nh, kout, s, odd); break;
+        case 6: launch_permute<T, 6>(kin, kp, nh, kout, s, odd); break;
+        case 7: launch_permute<T, 7>(kin, kp, nh, kout, s, odd); break;
+        case 8: launch_permute<T, 8>(kin, kp, nh, kout, s, odd); break;
+        case 9: launch_permute<T, 9>(kin, kp, nh, kout, s, odd); break;
+        case 10: launch_permute<T, 10>(kin, kp, nh, kout, s, odd); break;
+        case 11: launch_permute<T, 11>(kin, kp, nh, kout, s, odd); break;
+        case 12: launch_permute<T, 12>(kin, kp, nh, kout, s, odd); break;
+        case 13: launch_permute<T, 13>(kin, kp, nh, kout, s, odd); break;
         default: throw std::invalid_argument("column transform length out of range");
     }
 }
-template void rb_permute_table<float>(int, const float*, int, int, float*, hipStream_t);
-template void rb_permute_table<double>(int, const double*, int, int, double*, hipStream_t);
+template void rb_permute_table<float>(int, const float*, int, int, float*, hipStream_t, bool);
+template void rb_permute_table<double>(int, const double*, int, int, double*, hipStream_t,
+                                       bool);
 
 template void rb_rows_fwd<float>(int, const RbRowsFwd<float>&, hipStream_t);
 template void rb_rows_fwd<double>(int, const RbRowsFwd<double>&, hipStream_t);

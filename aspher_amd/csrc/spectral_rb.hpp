@@ -14,6 +14,7 @@ namespace aspher {
 struct RbBatch {
     int count = 1;
     std::int64_t x_stride = 0, y_stride = 0, at_stride = 0, br_stride = 0;
+    std::int64_t m_stride = -1;   // mask stride (-1: that of x, resp. y)
     int first = 0;
     int sstride = 0;
     std::int64_t pstride = 0;
@@ -29,7 +30,9 @@ struct RbRowsFwd {
     int pitchT = 0;
     int B = 1;                  // rows per workgroup
     const cx<double>* tw = nullptr;   // exp(-2 pi i t/n), n = 2L
-    int mode = 0;               // 0 plain, 1 masked (mask ? x - gmean : 0), 2 Love embedding
+    int mode = 0;               // 0 plain, 1 masked (mask ? x - gmean : 0),
+                                // 2/3/4 embedding of an even / transposed even /
+                                // odd-odd (xy) influence table
     const std::uint8_t* mask = nullptr;
     const double* scal = nullptr;
     int gmean_idx = 0;
@@ -92,9 +95,34 @@ void rb_gen_cols(int logN, const RbGen& a, hipStream_t s);
 template <typename T> void rb_rows_fwd(int logL, const RbRowsFwd<T>& a, hipStream_t s);
 template <typename T> void rb_cols(int logN, const RbCols<T>& a, hipStream_t s);
 template <typename T> void rb_rows_inv(int logL, const RbRowsInv<T>& a, hipStream_t s);
+// Two-component (tangential) column pass: instance 0 and 1 of the plan are
+// the x and y components; after the forward transform each mode is mixed by
+// a symmetric real 2x2 symbol
+//   sym 5: tables Kxx, Kyy, Kxy in the register order (free space, pruned)
+//   sym 6: |k|/Ns^2 (I + gamma k k^T/|k|^2), cross term 0 on Nyquist modes
+template <typename T>
+struct RbCols2 {
+    const cx<T>* AT = nullptr;
+    int pitchT = 0;
+    std::int64_t at_stride = 0;
+    cx<T>* BR = nullptr;
+    int pitch = 0;
+    std::int64_t br_stride = 0;
+    int ncols = 0, rows_in = 0, rows_out = 0;
+    const cx<double>* tw = nullptr;
+    int sym = 5;
+    const T *kxx = nullptr, *kyy = nullptr, *kxy = nullptr;
+    int kp = 0;
+    double gamma = 0.0, norm = 0.0;
+    int remap = 1;
+};
+template <typename T> void rb_cols2(int logN, const RbCols2<T>& a, hipStream_t s);
+
 // unique (n/2+1)^2 free-space spectrum -> nh x n table in the column
 // kernel's register order (see k_rb_permute_table)
+// (odd: the table is odd in ky, values for ky > n/2 take the opposite sign)
 template <typename T>
-void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s);
+void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s,
+                      bool odd = false);
 
 } // namespace aspher

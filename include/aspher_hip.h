@@ -293,6 +293,31 @@ int aspher_solve_ensemble_device(int n, const int* backend, const int* grid_size
                                  aspher_result** results, aspher_ensemble_stats* stats,
                                  double* instance_seconds);
 
+/* Tangential (Cerruti) 2x2 block operator u = [Kxx Kxy; Kxy Kyy] q on
+ * stacked [x; y] components of Ns^2 values each
+ * (TangentialFFTOperator, src/tangential_operator.cpp:16-157, with the
+ * CerrutiKernel tables of src/cerruti_kernel.cpp:117-127). The reference
+ * reaches it only through its friction driver; grid_size a power of two in
+ * [16, 2048]. */
+typedef struct aspher_tangential aspher_tangential;
+int aspher_tangential_create(int grid_size, double domain_size, double E_star, double nu,
+                             aspher_tangential** out);
+void aspher_tangential_destroy(aspher_tangential* t);
+/* host: q, u hold 2 Ns^2 doubles */
+int aspher_tangential_matvec(aspher_tangential* t, const double* q, double* u);
+/* device buffers of 2 Ns^2 FP64 (precision 64) or FP32 (32) values; may alias */
+int aspher_tangential_matvec_device(aspher_tangential* t, const void* q, void* u, int precision);
+/* TangentialFourierPreconditioner::apply_into (src/fourier_precond.cpp:291-358):
+ * g, z hold 2 Ns^2 doubles, mask Ns^2 bytes; precision 64 or 32. */
+int aspher_tangential_precond_apply(int grid_size, double nu, int precision, const double* g,
+                                    const uint8_t* mask, int remove_mean, double* z);
+/* Cerruti brackets cerruti_uxx / cerruti_uxy (src/cerruti_kernel.cpp:85-123)
+ * at n points, and the Ns x Ns scaled tables (tests of the device kernels). */
+int aspher_cerruti_points(int n, const double* x, const double* y, const double* a,
+                          const double* b, double nu, double* uxx, double* uxy);
+int aspher_cerruti_tables(int grid_size, double domain_size, double E_star, double nu,
+                          double* xx, double* xy);
+
 /* Self-affine Gaussian rough surface generated on the device. The
  * reference's benchmarks draw their surfaces from the third-party `rfgen`
  * package (pyproject.toml:15), outside the library; this replaces it with
