@@ -96,6 +96,9 @@ struct Result {
     std::string status_reason;
     double pk_error = 0.0, fw_gap = 0.0, fw_error = 0.0, load_error = 0.0;
     double pressure_violation = 0.0, penetration_error = 0.0;
+    // min over the solve's points of u + g0 in the solver's frame (the
+    // restricted certificate's vmin_local, src/contact_solver.cpp:1020-1030)
+    double vmin_local = 0.0;
     double requested_tol = 0.0, effective_tol = 0.0, p_ref = 0.0, g_ref = 0.0;
     std::string validation_scope = "solve_operator";
     std::string operator_error_kind = "unavailable";
@@ -298,6 +301,18 @@ struct NestedParams {
 
 bool stencil_for_level(const std::string& engine, bool precond, double prev_occupancy,
                        double occupancy_max);
+
+// One restricted (active-set) level of the nested driver
+// (src/nested_solve.cpp:117-516) on the free-space operator. glvl: the
+// level's FP64 gap (device), datum: the hierarchy's datum; p_init: the
+// prolonged pressure (Ns^2, consumed); coarse_gap: the gap field of the level
+// below ((Ns/2)^2). The result carries the global certificate.
+template <typename Real>
+void active_level(Context& ctx, Operator& op, Precond* pc, const double* glvl, double datum,
+                  double p_bar, double lvl_tol, int max_iter, bool use_pr,
+                  const NestedParams& np, DevBuf<double>& p_init,
+                  const DevBuf<double>& coarse_gap, bool record_history, bool light,
+                  double g_ref, Result& res);
 
 // gap: device FP64 Ns*Ns. Operators are built per level (or taken from the
 // process-wide cache when `reuse_operators` is on).

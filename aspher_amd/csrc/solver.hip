@@ -424,6 +424,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     res.load_error = d.load;
     res.pressure_violation = 0.0;
     res.penetration_error = d.pen;
+    res.vmin_local = d.gmin + d.alpha;
     res.requested_tol = req_tol;
     res.effective_tol = tol;
     res.p_ref = p_ref;
@@ -527,7 +528,10 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
     const bool any_float = policy != "double";
     if (!(np.float_floor > 0.0))
         throw std::invalid_argument("solve_contact_nested: float_floor must be positive");
-    if (policy == "float_then_double" && np.active_set)
+    // float_then_double with active_set: the reference rejects it for its h2
+    // backend; on the fft backend the float identification stage runs
+    // restricted and the double polish runs on the full grid.
+    if (policy == "float_then_double" && np.active_set && np.backend != "fft")
         throw std::invalid_argument(
             "solve_contact_nested: float_then_double does not yet route its polish through "
             "the active-set path; use float_only or double_only with active_set=true");
@@ -539,9 +543,12 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
         throw std::invalid_argument(
             "solve_contact_nested: backend must be 'h2', 'fft', or 'periodic'");
     if (np.active_set) {
-        if (np.backend != "h2")
+        // the restricted solve runs on the free-space operator (the
+        // reference's masked-matvec backend 'h2' is not part of this build)
+        if (np.backend != "fft")
             throw std::invalid_argument(
-                "solve_contact_nested: active_set requires backend='h2' (masked matvec)");
+                "solve_contact_nested: active_set requires backend='fft' (or the reference's "
+                "'h2', not part of this build)");
         if (levels.size() < 2)
             throw std::invalid_argument(
                 "solve_contact_nested: active_set needs a coarse level (Ns > coarsest)");
@@ -583,8 +590,9 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
     }
 
     DevBuf<double> p_init;
+    DevBuf<double> coarse_gap;   // gap field of the level below (active_set only)
     bool have_init = false;
-    double t_build = 0.0, t_coarse = 0.0;
+    double t_build = 0.0, t_coarse = 0.0, t_candidate_all = 0.0;
     double prev_occupancy = -1.0;
     std::vector<Stage> stages;
     auto record_stage = [&](const std::string& name, const char* prec, double req, double eff,
@@ -636,8 +644,18 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
         double lvl_tol = finest ? tol : np.coarse_tol;
         if (polish_after) lvl_tol = np.float_floor;
         else if (level_float) lvl_tol = std::max(lvl_tol, np.float_floor);
-        const bool light = finest ? np.light_result : true;
+        // a level whose successor will be restricted keeps its gap field:
+        // the successor's candidate set is built from it. Whether it will be
+        // restricted is predicted from this level's predecessor occupancy.
+        const bool gate_open_next =
+            prev_occupancy < 0.0 || prev_occupancy < np.active_occupancy_max;
+        const bool keep_gap = np.active_set && !finest && gate_open_next &&
+                              (np.active_all_levels || li + 2 == levels.size());
+        const bool light = finest ? np.light_result : !keep_gap;
         const bool record_history = finest && np.record_error_history;
+        const bool use_active = np.active_set && have_init && coarse_gap.p != nullptr &&
+                                prev_occupancy < np.active_occupancy_max &&
+                                (finest || np.active_all_levels);
 
         SolveOptions lopt;
         lopt.tol = lvl_tol;
@@ -654,7 +672,21 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
         lopt.precond_cost_gate = np.precond_cost_gate;
 
         Result lres;
-        if (level_float) {
+        if (use_active) {
+            if (level_float) {
+                op->build_single(s);
+                active_level<float>(ctx, *op, &pc, glvl, datum, p_bar, lvl_tol, max_iter, use_pr,
+                                    np, p_init, coarse_gap, record_history, light, g_ref, lres);
+            } else {
+                active_level<double>(ctx, *op, &pc, glvl, datum, p_bar, lvl_tol, max_iter,
+                                     use_pr, np, p_init, coarse_gap, record_history, light,
+                                     g_ref, lres);
+            }
+            if (!finest) gap[li].reset();
+            coarse_gap.reset();
+            lres.requested_tol = lopt.requested_tol;
+            t_candidate_all += lres.time_candidate;
+        } else if (level_float) {
             op->build_single(s);
             // the datum is removed inside the cast, before any rounding
             DevBuf<float> g0f(nn);
@@ -674,7 +706,8 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
             if (!finest) gap[li].reset();
         }
         prev_occupancy = lres.contact_fraction;
-        record_stage(std::string(finest ? "finest:" : "coarse:") + std::to_string(n),
+        record_stage(std::string(finest ? "finest:" : "coarse:") + std::to_string(n) +
+                         (use_active ? "(active)" : ""),
                      level_float ? "float" : "double", lopt.requested_tol, lopt.tol, lres);
 
         if (polish_after) {
@@ -717,6 +750,8 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
             p_init.alloc((std::size_t)4 * nn);
             prolong2(n, lres.pressure.p, p_init.p, s);
             have_init = true;
+            if (keep_gap && lres.gap.p) coarse_gap = std::move(lres.gap);
+            else coarse_gap.reset();
         }
         if (finest) res = std::move(lres);
     }
@@ -724,6 +759,7 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
     res.stage_stats = std::move(stages);
     res.time_build = t_build;
     res.time_coarse = t_coarse;
+    res.time_candidate = t_candidate_all;
     res.time_total = since(t_nested);
 }
 

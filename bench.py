@@ -67,9 +67,9 @@ def profiled_traffic(kernel):
     return None
 
 
-def run_solve(hc, Ns, gptr, precision, reuse, float_floor):
+def run_solve(hc, Ns, gptr, precision, reuse, float_floor, active=False):
     kw = dict(coarsest=64, backend="fft", precision=precision, light_result=False,
-              reuse_operators=reuse)
+              reuse_operators=reuse, active_set=active)
     if float_floor is not None:
         kw["float_floor"] = float_floor
     return hc.solve_nested_device(Ns, gptr, 0.005, tol=1e-6, **kw)
@@ -258,9 +258,14 @@ def main():
     t_mixed_so, _ = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff),
                           args.steps, 1)
     t_fp64_so, _ = timed(lambda: run_solve(hc, Ns, g.ptr, "double", True, ff), args.steps, 1)
-    hc.clear_operator_cache()
     log("[bench]   mixed %.3f s, fp64 %.3f s" % (t_mixed_so, t_fp64_so))
-    for r in (r_mixed, r_fp64):
+    log("[bench] mixed with active set (operators cached)")
+    t_act_so, r_act = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff,
+                                              active=True), args.steps, 1)
+    hc.clear_operator_cache()
+    log("[bench]   %.3f s  %s rounds %d fallback %s" % (t_act_so, r_act, r_act.active_rounds,
+                                                     r_act.active_fallback))
+    for r in (r_mixed, r_fp64, r_act):
         assert r.status == "converged", r
 
     pm, pd = r_mixed.pressure.ravel(), r_fp64.pressure.ravel()
@@ -271,6 +276,13 @@ def main():
         pressure_rel_l2=float(np.linalg.norm(pm - pd) / np.linalg.norm(pd)),
         mean_gap_mixed=float(gm.mean()), mean_gap_fp64=float(gd.mean()),
         fw_error_mixed=r_mixed.fw_error, penetration_mixed=r_mixed.penetration_error)
+    active = dict(solve_only_s=round(t_act_so, 4), iterations=r_act.iterations,
+                  rounds=r_act.active_rounds, fallback=bool(r_act.active_fallback),
+                  contact_area=r_act.contact_area,
+                  pressure_rel_l2_vs_fp64=float(np.linalg.norm(r_act.pressure.ravel() - pd)
+                                                / np.linalg.norm(pd)),
+                  stages=[[s_["name"], s_["precision"], s_["iterations"]]
+                          for s_ in r_act.stage_stats])
 
     # ── roofline of the dominant launch of the FP32 matvec ─────────────────
     m32 = mv[32]
@@ -357,6 +369,7 @@ def main():
                        "fp64_stages": [(s["name"], s["iterations"])
                                        for s in r_fp64.stage_stats]},
         "accuracy": accuracy,
+        "active_set": active,
         "matvec": {"fp32_ms": round(mv[32]["ms"], 4), "fp32_gbs": round(mv[32]["gbs"], 1),
                    "fp64_ms": round(mv[64]["ms"], 4), "fp64_gbs": round(mv[64]["gbs"], 1),
                    "fp32_stage_ms": [round(t, 4) for t in mv[32]["stages_ms"]],

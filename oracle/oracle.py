@@ -26,6 +26,16 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
   solve_contact            manufactured-KKT exact p*, support enumeration,
                            Westergaard closed form, FW bound identities
   solve_nested             nested == single level, datum invariance
+  solve_contact_active     restricted certificate is not global (paraboloid,
+                           half-domain candidates: tests/test_active.cpp:513-561),
+                           stays zero outside C, equals the full solve on the
+                           gap lifted outside C, equals the full solve when C
+                           holds the contact set
+  dilate_mask              scipy.ndimage.maximum_filter (square, clipped)
+  active_level             active nested == standard nested, fallback and
+                           expansion regressions, all-levels / finest-only
+                           stage names, occupancy gate, third-exit fallback
+                           (tests/test_active.cpp:309-511)
   kkt_gradient             central finite differences through the exact
                            support-enumeration solver and solve_contact
   circulant_matrix         equals the operator's matvec on random vectors
@@ -401,6 +411,9 @@ class Result:
     p_ref: float = 0.0
     g_ref: float = 0.0
     matvec_count: int = 0
+    verification_matvec_count: int = 0
+    active_rounds: int = 0
+    active_fallback: bool = False
     precond_count: int = 0
     identification_steps: int = 0
     returned_best: bool = False
@@ -690,6 +703,382 @@ def solve_contact(S: Callable, g0: np.ndarray, p_bar, opt: SolveOptions = None,
     return res
 
 
+# ── restricted (active-set) PK on a candidate set ───────────────────────────
+
+def solve_contact_active(S: Callable, g0: np.ndarray, p_bar, idx, opt: SolveOptions = None,
+                         precond: Optional[Callable] = None, p_init=None, n_grid: int = 0,
+                         dtype=np.float64):
+    """Restatement of solve_contact_active_impl<Real> (src/contact_solver.cpp:601-1040).
+
+    The PK iteration of `solve_contact` with every per-iteration O(N) loop
+    driven by the candidate index list `idx`: the pressure lives on idx only
+    and every reduction, update and the identification step's projection
+    range over idx. S and precond act on full-length vectors. Returns
+    (Result, certificate) with the certificate of :1020-1030 (pv_sum,
+    vmin_local, P_total, g_ref) in the frame of `g0`."""
+    opt = opt or SolveOptions()
+    R = np.dtype(dtype).type
+    is_double = (np.dtype(dtype) == np.float64)
+    g0 = np.ascontiguousarray(g0, dtype=dtype).ravel()
+    idx = np.asarray(idx, dtype=np.int64).ravel()
+    N, Nc = g0.size, idx.size
+    p_bar = R(p_bar)
+    if N == 0 or Nc == 0 or not (p_bar > 0):
+        raise ValueError("solve_contact_active: empty gap/candidate set or p_bar <= 0")
+    if not (opt.tol > 0.0):
+        raise ValueError("solve_contact_active: tol must be positive")
+    if opt.max_iter < 0:
+        raise ValueError("solve_contact_active: max_iter must be >= 0")
+    if not np.all(np.isfinite(g0)):
+        raise ValueError("solve_contact_active: g0 contains non-finite values")
+    N_grid = n_grid if n_grid > 0 else N
+    P_total = float(p_bar) * N_grid
+    gmin_v, gmax_v = float(g0.min()), float(g0.max())
+    g_ref = opt.g_ref
+    if not (g_ref > 0.0):
+        g_ref = gmax_v - gmin_v
+    if not (g_ref > 0.0):
+        g_ref = 1.0
+    if opt.datum_mode == "automatic":
+        datum = gmin_v + 0.5 * (gmax_v - gmin_v); subtract = True
+    elif opt.datum_mode == "solver":
+        datum = opt.datum; subtract = True
+    else:
+        datum = opt.datum; subtract = False
+    gshift = R(datum) if subtract else R(0)
+    p_ref = opt.p_ref if opt.p_ref > 0.0 else float(p_bar)
+    load_tol = opt.load_tol if opt.load_tol > 0.0 else (1e-12 if is_double else 5e-7)
+    tol = opt.tol
+    req_tol = opt.requested_tol if opt.requested_tol > 0.0 else tol
+    p_unif = R(P_total / Nc)
+    g0c = (g0[idx] - gshift).astype(dtype)            # (g0(i) - gshift) on idx
+
+    p = np.zeros(N, dtype=dtype)
+    if p_init is not None:
+        pi0 = np.asarray(p_init, dtype=dtype).ravel()
+        if pi0.size != N:
+            raise ValueError("solve_contact_active: p_init size mismatch")
+        p[idx] = np.maximum(pi0[idx], R(0))
+        s = float(p[idx].astype(np.float64).sum())
+        if s > 0.0:
+            p[idx] = (p[idx] * R(P_total / s)).astype(dtype)
+        else:
+            p[idx] = p_unif
+    else:
+        p[idx] = p_unif
+    t = np.zeros(N, dtype=dtype)
+    u = np.zeros(N, dtype=dtype)
+    g = np.zeros(N, dtype=dtype)
+    g_prev = np.zeros(N, dtype=dtype)
+    r = np.zeros(N, dtype=dtype)
+    contact = np.zeros(N, dtype=bool)
+    best_p = None
+    res = Result()
+    cnt = {"mv": 0, "pc": 0}
+    st = {"G_old": 1.0, "delta": R(0), "s_id": 0.0, "u_fresh": False}
+    t_start = time.perf_counter()
+    f64 = lambda a: a.astype(np.float64)
+
+    def apply(x):
+        cnt["mv"] += 1
+        return np.asarray(S(x), dtype=dtype).ravel()
+
+    def evaluate():
+        d = _Diag()
+        v = (u[idx] + g0c).astype(dtype)
+        pc_ = p[idx]
+        pos = pc_ > 0
+        vd = f64(v)
+        vmin = float(vd.min())
+        nc = int(pos.sum())
+        gsum = float(vd[pos].sum())
+        d.nc = nc
+        d.alpha = gsum / nc if nc else vmin
+        d.gmin = vmin - d.alpha
+        alpha, gmin = R(d.alpha), R(d.gmin)
+        pd = f64(pc_)
+        pv = float((pd * vd).sum())
+        gc = (v - alpha).astype(dtype)
+        g[idx] = gc
+        gd = f64(gc)
+        gcs = float(gd[pos].sum())
+        e = float((pd * np.abs(gd)).sum())
+        Gc = float((pd * f64((gc - gmin).astype(dtype))).sum())
+        psum = float(pd.sum())
+        if nc > 0:
+            dl = gcs / nc
+            d.alpha += dl
+            d.gmin -= dl
+        d.pk = e / (P_total * g_ref)
+        d.G = Gc
+        d.fw = Gc / (P_total * g_ref)
+        d.pen = max(0.0, -d.gmin) / g_ref
+        d.psum = psum
+        d.pv = pv
+        d.load = abs(psum - P_total) / P_total
+        d.finite = all(math.isfinite(q) for q in (gsum, vmin, psum, e, Gc))
+        return d
+
+    def identification_step():
+        nonlocal p, u, r, t, g_prev
+        pd = f64(p[idx])
+        f_cur = float((pd * (0.5 * f64(u[idx]) + f64(g0c))).sum())
+        if not (st["s_id"] > 0.0):
+            pp = float((pd * pd).sum())
+            pu = float((pd * f64(u[idx])).sum())
+            st["s_id"] = pp / pu if (pu > 0.0 and pp > 0.0) else 1.0
+        for trial in range(12):
+            s = R(st["s_id"])
+            y = (p[idx] - s * g[idx]).astype(dtype)
+            project_load_simplex(y, P_total)
+            z = np.zeros(N, dtype=dtype)
+            z[idx] = y
+            rr = apply(z)
+            zd = f64(z[idx])
+            f_try = float((zd * (0.5 * f64(rr[idx]) + f64(g0c))).sum())
+            gd = float((f64(g[idx]) * f64((z[idx] - p[idx]).astype(dtype))).sum())
+            if math.isfinite(f_try) and gd < 0.0 and f_try <= f_cur + 1e-4 * gd:
+                p = z
+                u, r = rr, u
+                st["u_fresh"] = True
+                st["delta"] = R(0)
+                t = np.zeros(N, dtype=dtype)
+                g_prev = np.zeros(N, dtype=dtype)
+                st["G_old"] = 1.0
+                res.identification_steps += 1
+                if trial == 0:
+                    st["s_id"] *= 2.0
+                return True
+            st["s_id"] *= 0.25
+        return False
+
+    status, reason = "max_iterations", ""
+    best_merit = seen_merit = math.inf
+    stall = forced_id = 0
+    stall_limit = opt.stall_limit if opt.stall_limit > 0 else 200
+    d = None
+    it = 0
+    while it < opt.max_iter:
+        if not st["u_fresh"]:
+            u = apply(p)
+        st["u_fresh"] = False
+        d = evaluate()
+        if not d.finite:
+            status = "nonfinite"
+            break
+        if opt.record_history:
+            res.error_history.append(d.pk)
+        if d.load <= load_tol and d.fw <= tol and d.pen <= tol:
+            status = "converged"
+            st["u_fresh"] = True
+            break
+        merit = max(d.fw, d.pen)
+        if opt.keep_best and merit < best_merit:
+            best_merit = merit
+            best_p = p.copy()
+        if merit < seen_merit * (1.0 - 1e-4):
+            seen_merit = merit
+            stall = 0
+        else:
+            stall += 1
+            if stall >= stall_limit:
+                if forced_id < 3 and identification_step():
+                    forced_id += 1
+                    stall = 0
+                    it += 1
+                    continue
+                status, reason = "stagnated", "stall"
+                break
+        did_cg = False
+        if d.nc >= 2:
+            contact[idx] = p[idx] > 0
+            if precond is not None:
+                z = np.asarray(precond(g, contact), dtype=dtype).ravel()
+                cnt["pc"] += 1
+            else:
+                z = np.zeros(N, dtype=dtype)
+                z[idx] = np.where(contact[idx], g[idx], R(0))
+            ci = idx[contact[idx]]
+            gci = f64(g[ci])
+            G = float((f64(z[ci]) * gci).sum())
+            G_pr = float((f64(z[ci]) * f64((g[ci] - g_prev[ci]).astype(dtype))).sum())
+            gg = float((gci * gci).sum())
+            G_old = st["G_old"]
+            beta_val = ((max(0.0, G_pr / G_old) if opt.use_pr else G / G_old)
+                        if G_old != 0.0 else 0.0)
+            beta = st["delta"] * R(beta_val)
+            t[idx] = np.where(contact[idx], z[idx] + beta * t[idx], R(0)).astype(dtype)
+            g_prev[idx] = g[idx]
+            st["G_old"] = G
+            r = apply(t)
+            rsum = float(f64(r[ci]).sum())
+            num = float((f64(g[ci]) * f64(t[ci])).sum())
+            rmean = R(rsum / d.nc if d.nc else 0.0)
+            den = float((f64((r[ci] - rmean).astype(dtype)) * f64(t[ci])).sum())
+            if den > 0.0 and math.isfinite(den) and math.isfinite(num):
+                tau = R(num / den)
+                tau_ov = R(num * num / (den * gg)) if gg > 0.0 else tau
+                pc_ = p[idx]
+                was = pc_ > 0
+                pi = np.where(was, np.maximum(pc_ - tau * t[idx], R(0)), pc_).astype(dtype)
+                act = (pi == 0) & (g[idx] < 0)
+                pi = np.where(act, pi - tau_ov * g[idx], pi).astype(dtype)
+                changed = bool(np.any((pi > 0) != was))
+                p[idx] = pi
+                st["delta"] = R(0) if changed else R(1)
+                total = float(f64(p[idx]).sum())
+                if total > 0.0:
+                    p[idx] = (p[idx] * R(P_total / total)).astype(dtype)
+                else:
+                    p[idx] = p_unif
+                did_cg = True
+        if not did_cg and not identification_step():
+            status = "nonpositive_curvature"
+            break
+        it += 1
+
+    if status != "converged" and opt.keep_best and best_p is not None:
+        if not (max(d.fw, d.pen) <= best_merit):
+            p = best_p
+            st["u_fresh"] = False
+            res.returned_best = True
+    if not st["u_fresh"]:
+        u = apply(p)
+    d = evaluate()
+    if not d.finite and status != "nonfinite":
+        status = "nonfinite"
+    if status == "converged" and not (d.load <= load_tol and d.fw <= tol and d.pen <= tol):
+        status, reason = "verification_failed", "final_check"
+    if status == "converged" and req_tol < tol and not opt.allow_tolerance_relaxation:
+        if not (d.fw <= req_tol and d.pen <= req_tol):
+            status, reason = "stagnated", "precision_limit"
+    obj = float((f64(p[idx]) * (0.5 * f64(u[idx]) + f64(g0c))).sum())
+    res.approach = d.alpha + datum
+    res.objective = obj + datum * P_total
+    res.iterations = it
+    res.contact_fraction = d.nc / N_grid
+    res.mean_pressure = d.psum / N_grid
+    res.status, res.status_reason = status, reason
+    res.converged = status == "converged"
+    res.error = res.pk_error = d.pk
+    res.fw_gap, res.fw_error = d.G, d.fw
+    res.load_error, res.penetration_error = d.load, d.pen
+    res.requested_tol, res.effective_tol = req_tol, tol
+    res.p_ref, res.g_ref = p_ref, g_ref
+    res.matvec_count, res.precond_count = cnt["mv"], cnt["pc"]
+    res.time_total = time.perf_counter() - t_start
+    res.pressure = f64(p)
+    cert = dict(pv_sum=d.pv, vmin_local=d.gmin + d.alpha, P_total=P_total, g_ref=g_ref)
+    return res, cert
+
+
+def dilate_mask(m: np.ndarray, Ns: int, r: int) -> np.ndarray:
+    """Chebyshev (square) dilation by radius r, window clipped at the grid
+    edge, separable: rows then columns (src/nested_solve.cpp:76-101)."""
+    M = np.asarray(m, dtype=bool).reshape(Ns, Ns)
+    if r <= 0:
+        return M.ravel().copy()
+    H = np.zeros_like(M)
+    for dx in range(-r, r + 1):
+        if dx >= 0:
+            H[:, :Ns - dx] |= M[:, dx:]
+        else:
+            H[:, -dx:] |= M[:, :Ns + dx]
+    V = np.zeros_like(M)
+    for dy in range(-r, r + 1):
+        if dy >= 0:
+            V[:Ns - dy, :] |= H[dy:, :]
+        else:
+            V[-dy:, :] |= H[:Ns + dy, :]
+    return V.ravel()
+
+
+def active_level(mv: Callable, precond: Optional[Callable], glvl, datum, p_bar, lvl_tol,
+                 max_iter, use_pr, Ns, p_init_d, coarse_gap, light, g_ref, dtype,
+                 active_delta=0.05, active_halo=2, active_max_rounds=5,
+                 record_history=False) -> Result:
+    """One restricted level (src/nested_solve.cpp:117-516) on a full-grid
+    operator: candidate set C = dilate(p_init > 0) | {coarse gap < delta},
+    restricted solve at tol/2, global verification with a fresh apply of the
+    returned pressure (G = G_local + P (vmin_local - vmin_global)), extension
+    by dilated violations, full-solve fallback. `glvl` is the level gap in
+    double, `mv` applies the operator in `dtype`."""
+    glvl = np.ascontiguousarray(glvl, dtype=np.float64).ravel()
+    N = Ns * Ns
+    R = np.dtype(dtype).type
+    gmax, gmin = float(glvl.max()), float(glvl.min())
+    g_scale = gmax - gmin if gmax > gmin else 1.0
+    delta = active_delta * g_scale
+    vthresh = -lvl_tol * g_scale
+    cmask = dilate_mask(np.asarray(p_init_d) > 0.0, Ns, active_halo)
+    cg = np.asarray(coarse_gap, dtype=np.float64).reshape(Ns // 2, Ns // 2)
+    cmask |= (np.repeat(np.repeat(cg, 2, axis=0), 2, axis=1) < delta).ravel()
+    g0w = (glvl - datum).astype(dtype)                 # centred working gap
+    p0 = np.asarray(p_init_d, dtype=np.float64).astype(dtype)
+    it_total = verify_mv = rounds = 0
+    certified = False
+    hist = []
+    res = None
+    pr = None
+    while rounds < active_max_rounds:
+        rounds += 1
+        aopt = SolveOptions(tol=0.5 * lvl_tol, requested_tol=lvl_tol, max_iter=max_iter,
+                            use_pr=use_pr, record_history=record_history,
+                            allow_tolerance_relaxation=True, g_ref=g_ref,
+                            datum_mode="caller", datum=datum)
+        res, cert = solve_contact_active(mv, g0w, R(p_bar), np.flatnonzero(cmask), aopt,
+                                         precond, p0, n_grid=N, dtype=dtype)
+        p0 = None
+        it_total += res.iterations
+        hist += list(res.error_history)
+        pr = res.pressure.astype(dtype)
+        a = res.approach - datum
+        v = np.asarray(mv(pr), dtype=dtype).astype(np.float64) + g0w.astype(np.float64)
+        verify_mv += 1
+        vmin_glob = float(v.min())
+        viol = (~cmask) & (v - a < vthresh)
+        nviol = int(viol.sum())
+        G_glob = res.fw_gap + cert["P_total"] * (cert["vmin_local"] - vmin_glob)
+        fw_glob = G_glob / (cert["P_total"] * cert["g_ref"])
+        pen_glob = max(0.0, -(vmin_glob - a)) / cert["g_ref"]
+        res.fw_gap, res.fw_error, res.penetration_error = G_glob, fw_glob, pen_glob
+        res.effective_tol = lvl_tol
+        if res.status == "converged" and nviol == 0 and fw_glob <= lvl_tol \
+                and pen_glob <= lvl_tol:
+            certified = True
+            break
+        if nviol == 0:
+            break
+        cmask |= dilate_mask(viol, Ns, active_halo)
+        p0 = pr
+    if not certified:
+        pf = p0 if p0 is not None else pr
+        double = np.dtype(dtype) == np.float64
+        fopt = SolveOptions(tol=lvl_tol, max_iter=max_iter, use_pr=use_pr, light=light,
+                            record_history=record_history, keep_best=not light,
+                            allow_tolerance_relaxation=True, g_ref=g_ref,
+                            datum_mode="solver" if double else "caller", datum=datum)
+        res = solve_contact(mv, glvl if double else g0w, R(p_bar), fopt, precond, pf,
+                            dtype=dtype)
+        it_total += res.iterations
+        hist += list(res.error_history)
+        res.active_fallback = True
+    elif not light:
+        a = res.approach - datum
+        disp = np.asarray(mv(pr), dtype=dtype).astype(np.float64)
+        res.displacement = disp
+        res.gap = disp + g0w.astype(np.float64) - a
+    res.active_rounds = rounds
+    res.iterations = it_total
+    res.verification_matvec_count = verify_mv
+    if not certified and not res.active_fallback and res.status == "converged":
+        res.status, res.status_reason = "verification_failed", "global_check"
+    res.converged = res.status == "converged"
+    if record_history:
+        res.error_history = hist
+    return res
+
+
 # ── L6: nested driver, fft / periodic branches (src/nested_solve.cpp) ───────
 
 def restrict_field(f: np.ndarray, Ns: int) -> np.ndarray:
@@ -710,10 +1099,15 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
                  precision="double", float_floor=2e-6,
                  allow_tolerance_relaxation=False, light_result=False,
                  backend="fft", record_error_history=False, leaf_side=8,
+                 active_set=False, active_delta=0.05, active_halo=2, active_max_rounds=5,
+                 active_all_levels=True, active_occupancy_max=0.4,
                  operators: dict = None) -> Result:
     """solve_contact_nested for backend fft|periodic (src/nested_solve.cpp:518-990).
 
-    `operators` may map level size -> prebuilt operator (to reuse tables)."""
+    With active_set=True (fft backend) every level with a coarser one beneath
+    it whose predicted occupancy passes the gate is solved by `active_level`
+    (src/nested_solve.cpp:765-870). `operators` may map level size -> prebuilt
+    operator (to reuse tables)."""
     g0 = np.ascontiguousarray(g0, dtype=np.float64).ravel()
     if g0.size != Ns * Ns:
         raise ValueError("g0 size != Ns*Ns")
@@ -725,6 +1119,12 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
     if not levels or levels[-1] != Ns:
         raise ValueError("Ns must equal coarsest * 2^k")
     any_float = precision != "double"
+    if active_set:
+        if backend != "fft":
+            raise ValueError("solve_contact_nested: active_set requires backend='fft'")
+        if len(levels) < 2:
+            raise ValueError("solve_contact_nested: active_set needs a coarse level "
+                             "(Ns > coarsest)")
     ncoarse = len(levels) - 1
     gap = [None] * ncoarse
     if ncoarse > 0:
@@ -735,6 +1135,7 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
     datum = gmn + 0.5 * (gmx - gmn)
     g_ref = max(gmx - gmn, p_bar * L / E_star)
     p_init = None
+    coarse_gap = None
     prev_occ = -1.0
     stages = []
     res = None
@@ -762,7 +1163,12 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
             lvl_tol = float_floor
         elif any_float:
             lvl_tol = max(lvl_tol, float_floor)
-        light = light_result if finest else True
+        gate_open_next = prev_occ < 0.0 or prev_occ < active_occupancy_max
+        keep_gap = (active_set and not finest and gate_open_next
+                    and (active_all_levels or li + 2 == len(levels)))
+        light = light_result if finest else (not keep_gap)
+        use_active = (active_set and p_init is not None and coarse_gap is not None
+                      and prev_occ < active_occupancy_max and (finest or active_all_levels))
         lopt = SolveOptions(tol=lvl_tol, requested_tol=(tol if finest else coarse_tol),
                             max_iter=max_iter, use_pr=use_pr, light=light,
                             record_history=finest and record_error_history,
@@ -774,7 +1180,14 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
         dt = np.float32 if any_float else np.float64
         pcf = (lambda gg, cc, _o=pc_obj, _d=dt: _o.apply(gg, cc, _d)) if pc_obj else None
         mvf = lambda x, _o=op, _d=dt: _o.matvec(x, _d)
-        if any_float:
+        if use_active:
+            res = active_level(mvf, pcf, glvl, datum, p_bar, lvl_tol, max_iter, use_pr, n,
+                               p_init, coarse_gap, light, g_ref, dt, active_delta,
+                               active_halo, active_max_rounds,
+                               record_history=finest and record_error_history)
+            res.requested_tol = lopt.requested_tol
+            coarse_gap = None
+        elif any_float:
             gf = (glvl - datum).astype(np.float32)
             res = solve_contact(mvf, gf, np.float32(p_bar), lopt, pcf,
                                 None if p_init is None else p_init.astype(np.float32),
@@ -782,7 +1195,8 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
         else:
             res = solve_contact(mvf, glvl, p_bar, lopt, pcf, p_init, dtype=np.float64)
         prev_occ = res.contact_fraction
-        stages.append(dict(name=("finest:" if finest else "coarse:") + str(n),
+        stages.append(dict(name=("finest:" if finest else "coarse:") + str(n)
+                           + ("(active)" if use_active else ""),
                            precision="float" if any_float else "double",
                            requested_tol=lopt.requested_tol, effective_tol=lopt.tol,
                            iterations=res.iterations, matvec_count=res.matvec_count,
@@ -811,6 +1225,7 @@ def solve_nested(Ns, g0, p_bar, L=1.0, E_star=1.0, coarsest=64, precond=True,
                 res.error_history = list(before.error_history) + list(res.error_history)
         if not finest:
             p_init = prolong_field(res.pressure, n)
+            coarse_gap = res.gap if keep_gap else None
     res.stage_stats = stages
     res.time_total = time.perf_counter() - t0
     return res
