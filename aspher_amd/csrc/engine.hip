@@ -350,6 +350,10 @@ Precond::Precond(Kind kind, int Ns, int radius) : kind_(kind), Ns_(Ns), R_(radiu
         if (radius < 1) throw std::invalid_argument("stencil radius must be >= 1");
         if (2 * radius + 1 > Ns)
             throw std::invalid_argument("stencil radius too large for the grid");
+        // the device stencil stages a (64 + 2R) x (16 + 2R) FP64 halo in LDS
+        if (radius > 26)
+            throw std::invalid_argument("stencil radius above 26 (the LDS halo of the device "
+                                        "stencil) is not supported");
         stencil_weights(Ns, radius, dx_, dy_, w_);
         std::vector<int> d(2 * dx_.size());
         for (std::size_t t = 0; t < dx_.size(); ++t) {

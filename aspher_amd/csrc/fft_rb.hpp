@@ -280,9 +280,19 @@ __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __rest
     }
 }
 
+// No-op hook of the exchanges.
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
 // Regroup registers from stage sa's grouping to stage sb's through LDS.
-template <typename T, int LOGN, int sa, int sb>
-__device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th) {
+// `hook` runs between the LDS writes and the barrier, where the registers
+// of the data are free: a caller issues global loads there (the kernel
+// spectrum of the column pass) so that their latency overlaps the barrier,
+// the LDS reads and the next stage.
+template <typename T, int LOGN, int sa, int sb, typename Hook = NoHook>
+__device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
+                                         const Hook& hook = Hook{}) {
     using Sh = Shape<LOGN>;
     constexpr int Na = Sh::N(sa), Ga = Sh::G(sa), Nb = Sh::N(sb), Gb = Sh::G(sb);
     __syncthreads();
@@ -290,6 +300,7 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th) {
     for (int q = 0; q < Ga; ++q)
 #pragma unroll
         for (int v = 0; v < Na; ++v) lds[lds_ix(Sh::pos(sa, th * Ga + q, v))] = r[q * Na + v];
+    hook();
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < Gb; ++q)
@@ -297,13 +308,17 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th) {
         for (int v = 0; v < Nb; ++v) r[q * Nb + v] = lds[lds_ix(Sh::pos(sb, th * Gb + q, v))];
 }
 
-template <typename T, int LOGN, int s, bool ZIN>
+template <typename T, int LOGN, int s, bool ZIN, typename Hook>
 __device__ __forceinline__ void forward_from(cx<T>* r, cx<T>* lds, int th,
-                                             const cx<double>* __restrict__ tw, int twmul) {
+                                             const cx<double>* __restrict__ tw, int twmul,
+                                             const Hook& hook) {
     stage<T, LOGN, s, false, (s == 0 && ZIN) ? 1 : 0>(r, th, tw, twmul);
     if constexpr (s + 1 < Shape<LOGN>::m) {
-        exchange<T, LOGN, s, s + 1>(r, lds, th);
-        forward_from<T, LOGN, s + 1, ZIN>(r, lds, th, tw, twmul);
+        if constexpr (s + 2 == Shape<LOGN>::m) exchange<T, LOGN, s, s + 1>(r, lds, th, hook);
+        else exchange<T, LOGN, s, s + 1>(r, lds, th);
+        forward_from<T, LOGN, s + 1, ZIN>(r, lds, th, tw, twmul, hook);
+    } else if constexpr (s == 0) {
+        hook();   // single-stage transform: no exchange to hide the loads in
     }
 }
 
@@ -321,10 +336,12 @@ __device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
 // grouping (digit-reversed); inverse: the reverse. ZIN: the upper half of
 // the input (positions >= n/2, i.e. v >= 8 of stage 0) is zero; HOUT: only
 // the lower half of the output is used.
-template <typename T, int LOGN, bool ZIN = false>
+// `hook` runs once, inside the last exchange (see `exchange`).
+template <typename T, int LOGN, bool ZIN = false, typename Hook = NoHook>
 __device__ __forceinline__ void forward(cx<T>* r, cx<T>* lds, int th,
-                                        const cx<double>* __restrict__ tw, int twmul) {
-    forward_from<T, LOGN, 0, ZIN>(r, lds, th, tw, twmul);
+                                        const cx<double>* __restrict__ tw, int twmul,
+                                        const Hook& hook = Hook{}) {
+    forward_from<T, LOGN, 0, ZIN>(r, lds, th, tw, twmul, hook);
 }
 template <typename T, int LOGN, bool HOUT = false>
 __device__ __forceinline__ void inverse(cx<T>* r, cx<T>* lds, int th,

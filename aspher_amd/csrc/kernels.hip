@@ -133,15 +133,19 @@ k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
 // (begin = tile index, row = partial row).
 constexpr int kStX = 64, kStY = 16;
 
-template <typename T>
+// RC > 0: the radius as a compile-time constant (halo shape and its index
+// arithmetic fold into constants, the staging loop unrolls and issues all
+// its loads together); RC = 0: radius R at run time.
+template <typename T, int RC>
 __global__ void __launch_bounds__(256)
 k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask,
           const double* __restrict__ scal, const int* __restrict__ tapd,
-          const double* __restrict__ tapw, int ntaps, int R, T* __restrict__ z,
+          const double* __restrict__ tapw, int ntaps, int Rrt, T* __restrict__ z,
           double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
     extern __shared__ __align__(16) unsigned char smem[];
     double* tile = reinterpret_cast<double*>(smem);
+    const int R = RC > 0 ? RC : Rrt;
     int t0, t1, ts, inst = 0, row = blockIdx.x;
     if (ens.work) {
         const WorkItem w = ens.work[blockIdx.x];
@@ -161,21 +165,45 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
     g += o; mask += o; z += o;
     scal += (std::size_t)inst * ens.sstride;
     double* pr = part + (std::size_t)inst * ens.pstride + (std::size_t)row * kPartVals;
-    const int HX = kStX + 2 * R, HY = kStY + 2 * R;
+    const int HX = kStX + 2 * R, HY = kStY + 2 * R, HN = HX * HY;
+    const float inv_hx = 1.0f / (float)HX;
     const double gmean = scal[S_GMEAN];
     const int ntx = (Ns + kStX - 1) / kStX;
     const int tx = threadIdx.x & 63, ty0 = threadIdx.x >> 6;
     double zsum = 0.0;
     for (int t = t0; t < t1; t += ts) {
         const int x0 = (t % ntx) * kStX, y0 = (t / ntx) * kStY;
+        // only the part of the halo that a grid point of this tile reads is
+        // staged (a tile may overhang a grid smaller than itself); there the
+        // periodic wrap is one add or subtract, since 2R + 1 <= Ns
+        const int hxv = min(kStX, Ns - x0) + 2 * R, hyv = min(kStY, Ns - y0) + 2 * R;
         __syncthreads();
-        for (int e = threadIdx.x; e < HX * HY; e += blockDim.x) {
-            const int hy = e / HX, hx = e - hy * HX;
-            int jx = (x0 - R + hx) % Ns, jy = (y0 - R + hy) % Ns;
-            if (jx < 0) jx += Ns;
-            if (jy < 0) jy += Ns;
-            const std::size_t j = (std::size_t)jy * Ns + jx;
-            tile[e] = mask[j] ? (double)g[j] - gmean : 0.0;
+        auto stage_one = [&](int e, int hy) {
+            const int hx = e - hy * HX;
+            if (hx < hxv && hy < hyv) {
+                int jx = x0 - R + hx, jy = y0 - R + hy;
+                jx += jx < 0 ? Ns : 0;
+                jx -= jx >= Ns ? Ns : 0;
+                jy += jy < 0 ? Ns : 0;
+                jy -= jy >= Ns ? Ns : 0;
+                const std::size_t j = (std::size_t)jy * Ns + jx;
+                const T gv = g[j];
+                const std::uint8_t mv = mask[j];
+                tile[e] = mv ? (double)gv - gmean : 0.0;
+            }
+        };
+        if constexpr (RC > 0) {
+            constexpr int HXc = kStX + 2 * RC, HNc = HXc * (kStY + 2 * RC);
+#pragma unroll
+            for (int k = 0; k < (HNc + 255) / 256; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                if (e < HNc) stage_one(e, e / HXc);
+            }
+        } else {
+            // e / HX by the rounded reciprocal: exact here, since (e + 0.5)/HX
+            // is at least 0.5/HX away from an integer, far above float rounding
+            for (int e = threadIdx.x; e < HN; e += 256)
+                stage_one(e, (int)(((float)e + 0.5f) * inv_hx));
         }
         __syncthreads();
         const int ix = x0 + tx;
@@ -731,20 +759,66 @@ k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __re
     part += (std::size_t)inst * ens.pstride;
     scal += (std::size_t)inst * ens.sstride;
     if (ens.rows) nblk = ens.rows[inst];
-    for (int v = 0; v < spec.nv; ++v) {
-        const int op = spec.ops[v];
-        double acc = op == RSUM ? 0.0 : (op == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-            const double x = part[(std::size_t)b * kPartVals + v];
-            acc = op == RSUM ? acc + x : (op == RMIN ? ((x < acc) ? x : acc) : ((x > acc) ? x : acc));
+    // one pass over the partial rows for all values, then one combination
+    // of the waves: per value the same order as a reduction of that value
+    // alone (thread-strided rows, shuffle tree, waves in index order)
+    const int nv = spec.nv;
+    double acc[kPartVals];
+#pragma unroll
+    for (int v = 0; v < kPartVals; ++v)
+        acc[v] = spec.ops[v] == RSUM ? 0.0
+                 : (spec.ops[v] == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
+    // rows b, b + 256, ... in that order; eight rows are loaded before any is
+    // accumulated, so a batch costs one memory latency, not eight
+    constexpr int kBatch = 8;
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += kBatch * blockDim.x) {
+        double x[kBatch][kPartVals];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int b = b0 + u * (int)blockDim.x;
+            const double* row = part + (std::size_t)(b < nblk ? b : 0) * kPartVals;
+#pragma unroll
+            for (int v = 0; v < kPartVals; ++v) x[u][v] = v < nv ? row[v] : 0.0;
         }
-        double r;
-        if (op == RSUM) r = block_reduce<RSUM>(acc, red);
-        else if (op == RMIN) r = block_reduce<RMIN>(acc, red);
-        else r = block_reduce<RMAX>(acc, red);
-        if (threadIdx.x == 0) scal[spec.dst[v]] = r;
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (b0 + u * (int)blockDim.x >= nblk) break;
+#pragma unroll
+            for (int v = 0; v < kPartVals; ++v) {
+                if (v >= nv) break;
+                const double xv = x[u][v];
+                const int op = spec.ops[v];
+                acc[v] = op == RSUM ? acc[v] + xv
+                         : (op == RMIN ? ((xv < acc[v]) ? xv : acc[v])
+                                       : ((xv > acc[v]) ? xv : acc[v]));
+            }
+        }
     }
-    if (threadIdx.x == 0) run_step<T>(step, scal);
+    __shared__ double wred[8 * kPartVals];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int v = 0; v < kPartVals; ++v) {
+        if (v >= nv) break;
+        const int op = spec.ops[v];
+        acc[v] = op == RSUM ? wave_reduce<RSUM>(acc[v])
+                 : (op == RMIN ? wave_reduce<RMIN>(acc[v]) : wave_reduce<RMAX>(acc[v]));
+        if (lane == 0) wred[wave * kPartVals + v] = acc[v];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 0; v < nv; ++v) {
+            const int op = spec.ops[v];
+            double r = op == RSUM ? 0.0 : (op == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
+            for (int w = 0; w < nw; ++w) {
+                const double x = wred[w * kPartVals + v];
+                r = op == RSUM ? r + x : (op == RMIN ? ((x < r) ? x : r) : ((x > r) ? x : r));
+            }
+            scal[spec.dst[v]] = r;
+        }
+        run_step<T>(step, scal);
+    }
+    (void)red;
 }
 
 struct SetSpec {
@@ -1042,8 +1116,17 @@ void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const doub
                         const int* tapd, const double* tapw, int ntaps, int R, T* z,
                         double* part, int nblk, hipStream_t s, const Ens& ens) {
     const std::size_t lds = sizeof(double) * (kStX + 2 * R) * (kStY + 2 * R);
-    k_stencil<T><<<ens.work ? ens.nwork : nblk, kVecThreads, lds, s>>>(
-        Ns, g, mask, scal, tapd, tapw, ntaps, R, z, part, ens);
+    const int grid = ens.work ? ens.nwork : nblk;
+    switch (R) {
+#define ST_CASE(RR) \
+        case RR: k_stencil<T, RR><<<grid, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, \
+                                                               ntaps, R, z, part, ens); break;
+        ST_CASE(1) ST_CASE(2) ST_CASE(3) ST_CASE(4)
+#undef ST_CASE
+        default:
+            k_stencil<T, 0><<<grid, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, ntaps, R,
+                                                          z, part, ens);
+    }
     ASPHER_LAUNCH_CHECK();
 }
 template <typename T>

@@ -198,7 +198,16 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
         if (ZP && v >= 8) r[v] = {T(0), T(0)};
         else r[v] = (live && p < rows_in) ? col[p] : cx<T>{T(0), T(0)};
     }
-    rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1);
+    // the kernel table in the plan's digit-reversed layout: the 16 values of
+    // this thread are contiguous at kx*kp + 16*th (Gl*Nl = 16); loaded
+    // inside the last forward exchange so the HBM latency is hidden
+    T kv[16];
+    if constexpr (SYM == 0) {
+        const T* kr = ktab + (size_t)(live ? kx : 0) * kp + 16 * th;
+        rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1, [&]() { load16(kr, kv); });
+    } else {
+        rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1);
+    }
 
     if constexpr (SYM == 3) {
         // kernel-spectrum build (src/fft_operator.cpp:46-68): realness check
@@ -224,11 +233,6 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
         return;
     } else {
         if constexpr (SYM == 0) {
-            // table in the plan's digit-reversed layout: the 16 values of
-            // this thread are contiguous at kx*kp + 16*th (Gl*Nl = 16)
-            const T* kr = ktab + (size_t)(live ? kx : 0) * kp + 16 * th;
-            T kv[16];
-            load16(kr, kv);
 #pragma unroll
             for (int i = 0; i < 16; ++i) r[i] = scale(r[i], live ? kv[i] : T(0));
         } else {

@@ -289,8 +289,8 @@ def main():
     names = ("rows_fwd", "cols", "rows_inv")
     k = int(np.argmax(m32["stages_ms"]))
     ach = m32["bytes"][names[k]] / (m32["stages_ms"][k] * 1e-3) / 1e9
-    kname = {"rows_fwd": "k_rb_rows_fwd<float, 12, 0>", "cols": "k_rb_cols<float, 13, 0>",
-             "rows_inv": "k_rb_rows_inv<float, 12, 0>"}[names[k]]
+    kname = {"rows_fwd": "k_rb_rows_fwd<float, 12, 0, true>", "cols": "k_rb_cols<float, 13, 0>",
+             "rows_inv": "k_rb_rows_inv<float, 12, 0, true>"}[names[k]]
     traffic = profiled_traffic(kname) if Ns == 4096 else None
     roofline = dict(bound="hbm", kernel=names[k], achieved=round(ach, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic,

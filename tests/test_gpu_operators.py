@@ -207,6 +207,22 @@ def test_preconditioners_match_oracle(hc, Ns):
         assert np.linalg.norm(zf - rf) / np.linalg.norm(rf) < 1e-5, kind
 
 
+@pytest.mark.parametrize("Ns,radius", [(8, 1), (16, 2), (96, 3), (200, 2), (40, 4), (64, 5), (300, 7)])
+def test_stencil_small_and_ragged_grids(hc, Ns, radius):
+    """Grids smaller than one 64x16 tile and grids that are not a multiple of
+    it: the periodic halo wraps, the tile overhangs the grid."""
+    rng = np.random.default_rng(Ns + radius)
+    mask = rng.random(Ns * Ns) < 0.5
+    g = rng.uniform(-1, 1, Ns * Ns)
+    ref = O.StencilPreconditioner(Ns, radius)
+    z = _precond("fourier", Ns, g, mask, radius=radius)
+    r = ref.apply(g, mask)
+    assert np.linalg.norm(z - r) / np.linalg.norm(r) < 1e-12
+    zf = _precond("fourier", Ns, g, mask, precision=32, radius=radius)
+    rf = ref.apply(g.astype(np.float32), mask).astype(np.float64)
+    assert np.linalg.norm(zf - rf) / np.linalg.norm(rf) < 1e-5
+
+
 def test_preconditioners_offset_invariant_symmetric(hc):
     """tests/test_precond.cpp:27-66."""
     Ns = 32
