@@ -59,10 +59,12 @@ Context::Context() {
     part.alloc((std::size_t)kMaxPartRows * kPartVals);
     ASPHER_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_scal), sizeof(double) * S_COUNT));
     ASPHER_HIP(hipMemsetAsync(scal.p, 0, sizeof(double) * S_COUNT, stream));
+    ASPHER_HIP(hipEventCreateWithFlags(&scal_ready, hipEventDisableTiming));
 }
 
 Context::~Context() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (scal_ready) (void)hipEventDestroy(scal_ready);
     if (host_scal) (void)hipHostFree(host_scal);
     scal.reset();
     part.reset();
@@ -74,6 +76,14 @@ void Context::read_scal() {
                               hipMemcpyDeviceToHost, stream));
     ASPHER_HIP(hipStreamSynchronize(stream));
 }
+
+void Context::copy_scal_async() {
+    ASPHER_HIP(hipMemcpyAsync(host_scal, scal.p, sizeof(double) * S_COUNT,
+                              hipMemcpyDeviceToHost, stream));
+    ASPHER_HIP(hipEventRecord(scal_ready, stream));
+}
+
+void Context::wait_scal() { ASPHER_HIP(hipEventSynchronize(scal_ready)); }
 
 // ── Operator ───────────────────────────────────────────────────────────────
 

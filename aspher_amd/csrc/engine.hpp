@@ -264,6 +264,11 @@ struct Context {
     Context(const Context&) = delete;
     Context& operator=(const Context&) = delete;
     void read_scal();              // D2H of the scalar block + stream sync
+    // D2H of the scalar block without waiting: the stream runs on, and
+    // wait_scal() blocks only until that copy has landed
+    void copy_scal_async();
+    void wait_scal();
+    hipEvent_t scal_ready = nullptr;
 };
 
 // One certified PK solve (src/contact_solver.cpp:150-587) on device inputs:

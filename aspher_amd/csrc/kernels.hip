@@ -171,41 +171,45 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
     const int ntx = (Ns + kStX - 1) / kStX;
     const int tx = threadIdx.x & 63, ty0 = threadIdx.x >> 6;
     double zsum = 0.0;
-    for (int t = t0; t < t1; t += ts) {
+    // halo element e of tile t: the grid index it reads, or -1 where no grid
+    // point of the tile reads it (a tile may overhang a grid smaller than
+    // itself). Within the read part the periodic wrap is one add or
+    // subtract, since 2R + 1 <= Ns.
+    auto halo_index = [&](int t, int e, int hy) -> std::int64_t {
         const int x0 = (t % ntx) * kStX, y0 = (t / ntx) * kStY;
-        // only the part of the halo that a grid point of this tile reads is
-        // staged (a tile may overhang a grid smaller than itself); there the
-        // periodic wrap is one add or subtract, since 2R + 1 <= Ns
         const int hxv = min(kStX, Ns - x0) + 2 * R, hyv = min(kStY, Ns - y0) + 2 * R;
-        __syncthreads();
-        auto stage_one = [&](int e, int hy) {
-            const int hx = e - hy * HX;
-            if (hx < hxv && hy < hyv) {
-                int jx = x0 - R + hx, jy = y0 - R + hy;
-                jx += jx < 0 ? Ns : 0;
-                jx -= jx >= Ns ? Ns : 0;
-                jy += jy < 0 ? Ns : 0;
-                jy -= jy >= Ns ? Ns : 0;
-                const std::size_t j = (std::size_t)jy * Ns + jx;
-                const T gv = g[j];
-                const std::uint8_t mv = mask[j];
-                tile[e] = mv ? (double)gv - gmean : 0.0;
-            }
-        };
-        if constexpr (RC > 0) {
-            constexpr int HXc = kStX + 2 * RC, HNc = HXc * (kStY + 2 * RC);
+        const int hx = e - hy * HX;
+        if (hx >= hxv || hy >= hyv) return -1;
+        int jx = x0 - R + hx, jy = y0 - R + hy;
+        jx += jx < 0 ? Ns : 0;
+        jx -= jx >= Ns ? Ns : 0;
+        jy += jy < 0 ? Ns : 0;
+        jy -= jy >= Ns ? Ns : 0;
+        return (std::int64_t)jy * Ns + jx;
+    };
+    // taps: for a compile-time radius the l2 disc is enumerated here in the
+    // host's order (dy outer, dx inner, both ascending; the host checks the
+    // tap count), so the LDS offsets are constants of the unrolled loop; the
+    // weights are uniform (scalar registers for R <= 2, LDS broadcast
+    // reads otherwise). The summation order is the tap order either way.
+    constexpr int NT = RC == 1 ? 5 : RC == 2 ? 13 : RC == 3 ? 29 : RC == 4 ? 49 : 1;
+    __shared__ double s_w[RC >= 3 ? NT : 1];
+    double tw_r[RC >= 1 && RC <= 2 ? NT : 1];
+    if constexpr (RC >= 1 && RC <= 2) {
 #pragma unroll
-            for (int k = 0; k < (HNc + 255) / 256; ++k) {
-                const int e = threadIdx.x + 256 * k;
-                if (e < HNc) stage_one(e, e / HXc);
-            }
-        } else {
-            // e / HX by the rounded reciprocal: exact here, since (e + 0.5)/HX
-            // is at least 0.5/HX away from an integer, far above float rounding
-            for (int e = threadIdx.x; e < HN; e += 256)
-                stage_one(e, (int)(((float)e + 0.5f) * inv_hx));
+        for (int q = 0; q < NT; ++q) {
+            const double wq = tapw[q];
+            const unsigned long long b = __double_as_longlong(wq);
+            const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(b & 0xffffffffu));
+            const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+            tw_r[q] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
         }
-        __syncthreads();
+    } else if constexpr (RC >= 3) {
+        if ((int)threadIdx.x < NT) s_w[threadIdx.x] = tapw[threadIdx.x];
+        // the first tile's barrier orders these writes before any read
+    }
+    auto compute_tile = [&](int t) {
+        const int x0 = (t % ntx) * kStX, y0 = (t / ntx) * kStY;
         const int ix = x0 + tx;
 #pragma unroll
         for (int k = 0; k < kStY / 4; ++k) {
@@ -218,9 +222,65 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
             }
             double acc = 0.0;
             const double* c = tile + (ty + R) * HX + (tx + R);
-            for (int q = 0; q < ntaps; ++q) acc += tapw[q] * c[tapd[2 * q + 1] * HX + tapd[2 * q]];
+            if constexpr (RC > 0) {
+                constexpr int HXc = kStX + 2 * RC;
+                int q = 0;
+#pragma unroll
+                for (int dy = -RC; dy <= RC; ++dy)
+#pragma unroll
+                    for (int dx = -RC; dx <= RC; ++dx) {
+                        if (dx * dx + dy * dy > RC * RC) continue;
+                        const double wq = RC <= 2 ? tw_r[RC <= 2 ? q : 0] : s_w[RC >= 3 ? q : 0];
+                        acc += wq * c[dy * HXc + dx];
+                        ++q;
+                    }
+            } else {
+                for (int q = 0; q < ntaps; ++q)
+                    acc += tapw[q] * c[tapd[2 * q + 1] * HX + tapd[2 * q]];
+            }
             z[i] = static_cast<T>(acc);
             zsum += acc;
+        }
+    };
+    if constexpr (RC > 0) {
+        // software-pipelined: the halo of the next tile is loaded into
+        // registers while the current tile is computed
+        constexpr int HXc = kStX + 2 * RC, HNc = HXc * (kStY + 2 * RC);
+        constexpr int K = (HNc + 255) / 256;
+        T gv[K];
+        std::uint8_t mv[K];
+        auto fetch = [&](int t) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                const std::int64_t j = e < HNc ? halo_index(t, e, e / HXc) : -1;
+                gv[k] = j >= 0 ? g[j] : T(0);
+                mv[k] = j >= 0 ? mask[j] : std::uint8_t(0);
+            }
+        };
+        if (t0 < t1) fetch(t0);
+        for (int t = t0; t < t1; t += ts) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                if (e < HNc) tile[e] = mv[k] ? (double)gv[k] - gmean : 0.0;
+            }
+            if (t + ts < t1) fetch(t + ts);
+            __syncthreads();
+            compute_tile(t);
+        }
+    } else {
+        for (int t = t0; t < t1; t += ts) {
+            __syncthreads();
+            // e / HX by the rounded reciprocal: exact here, since (e + 0.5)/HX
+            // is at least 0.5/HX away from an integer, far above float rounding
+            for (int e = threadIdx.x; e < HN; e += 256) {
+                const std::int64_t j = halo_index(t, e, (int)(((float)e + 0.5f) * inv_hx));
+                if (j >= 0) tile[e] = mask[j] ? (double)g[j] - gmean : 0.0;
+            }
+            __syncthreads();
+            compute_tile(t);
         }
     }
     putr<RSUM>(zsum, red, pr, 0);
@@ -1117,7 +1177,9 @@ void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const doub
                         double* part, int nblk, hipStream_t s, const Ens& ens) {
     const std::size_t lds = sizeof(double) * (kStX + 2 * R) * (kStY + 2 * R);
     const int grid = ens.work ? ens.nwork : nblk;
-    switch (R) {
+    // the unrolled kernels assume the l2 disc's tap count of their radius
+    static const int disc_taps[5] = {0, 5, 13, 29, 49};
+    switch (R >= 1 && R <= 4 && ntaps == disc_taps[R] ? R : 0) {
 #define ST_CASE(RR) \
         case RR: k_stencil<T, RR><<<grid, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, \
                                                                ntaps, R, z, part, ens); break;

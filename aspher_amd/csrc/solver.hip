@@ -245,8 +245,10 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     bool precond_dropped = false;
 
     // one CG step: direction, second operator apply, line search, projected
-    // update and load rescale; the update is predicated on the device
-    auto launch_cg = [&]() {
+    // update and load rescale; the update is predicated on the device and
+    // writes out of place into the returned pressure buffer (`commit` makes
+    // it the iterate)
+    auto enqueue_cg = [&]() -> int {
         static const int dg[3] = {S_G, S_GPR, S_GG};
         static const int dl1[2] = {S_RSUM, S_NUM};
         static const int dl2[1] = {S_DEN};
@@ -284,8 +286,29 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         V::update(N, P, pb[o], T_, G, scal, part, nblk, s);
         finalize<Real>(part, nblk, 2, ou, du, scal, STEP_UPDATE, s);
         V::rescale(N, pb[o], scal, s);
+        return o;
+    };
+    auto commit = [&](int o) {
         cur = o;
         P = pb[o];
+    };
+
+    // The CG step of an iteration is enqueued before the host has read that
+    // iteration's diagnostics, so the device runs it while the host decides.
+    // It is kept when the host takes the CG branch (the common case). On any
+    // other branch it is dropped: the loop then ends (the step wrote only the
+    // CG state and a pressure buffer that is not the iterate or the best
+    // one), or an identification step follows, which resets that CG state
+    // (t, g_prev, delta, G_old) before use. The opt-in cost gate synchronises
+    // inside the step, so it runs without this overlap.
+    const bool speculate = !(opt.precond_cost_gate > 0.0);
+    int spec_o = -1;
+    long long spec_mv0 = 0, spec_pc0 = 0;
+    auto discard_spec = [&]() {
+        if (spec_o < 0) return;
+        mv = spec_mv0;   // a dropped step's operator and preconditioner
+        pcount = spec_pc0;   // applies are not counted
+        spec_o = -1;
     };
 
     Status status = Status::max_iterations;
@@ -303,13 +326,24 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         if (!u_fresh) apply(P, U);
         u_fresh = false;
         evaluate();
-        sync_read();
+        if (speculate) {
+            ctx.copy_scal_async();
+            spec_mv0 = mv;
+            spec_pc0 = pcount;
+            spec_o = enqueue_cg();
+            ctx.wait_scal();
+            tm_mv.drain();
+            tm_pc.drain();
+        } else {
+            sync_read();
+        }
         d = read_diag(hs);
         if (pending) {
             pending = false;
             if (hs[S_OK] == 0.0) {
                 // the previous step had no usable curvature: run its
                 // identification step now, on the identical state
+                discard_spec();
                 if (!identification_step()) {
                     status = Status::nonpositive_curvature;
                     --it;
@@ -322,11 +356,13 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             }
         }
         if (!d.finite) {
+            discard_spec();
             status = Status::nonfinite;
             break;
         }
         if (opt.record_history) res.error_history.push_back(d.pk);
         if (d.load <= load_tol && d.fw <= tol && d.pen <= tol) {
+            discard_spec();
             status = Status::converged;
             u_fresh = true;
             break;
@@ -341,6 +377,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             seen_merit = merit;
             stall = 0;
         } else if (++stall >= stall_limit) {
+            discard_spec();
             if (forced_id < 3 && identification_step()) {
                 ++forced_id;
                 stall = 0;
@@ -351,13 +388,22 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             break;
         }
         if (d.nc >= 2) {
-            launch_cg();
+            if (spec_o >= 0) {
+                commit(spec_o);
+                spec_o = -1;
+            } else {
+                commit(enqueue_cg());
+            }
             pending = true;
-        } else if (!identification_step()) {
-            status = Status::nonpositive_curvature;
-            break;
+        } else {
+            discard_spec();
+            if (!identification_step()) {
+                status = Status::nonpositive_curvature;
+                break;
+            }
         }
     }
+    discard_spec();
     if (pending) {
         // the loop ended on the iteration budget right after a CG step
         sync_read();

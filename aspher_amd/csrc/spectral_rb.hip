@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 namespace aspher {
 
@@ -163,8 +164,10 @@ __device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restric
     }
 }
 
+// FP32 table symbol: at most 128 VGPRs, so that two 512-thread workgroups
+// (4 waves per SIMD) stay resident, which the LDS of two columns allows
 template <typename T, int LOGN, int SYM>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, (sizeof(T) == 4 && SYM == 0) ? 4 : 1)
 k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int pitch,
           int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
           const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
@@ -181,11 +184,20 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
     constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red[16];
-    const int tile = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int c = threadIdx.x / TPC, th = threadIdx.x - c * TPC;
+    const int tile0 = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    // tiles beyond the grid (the host folds a small last round, e.g. the
+    // Nyquist column of an (Ns+1)-column half spectrum, into the first
+    // blocks) are taken in a second pass; the pass condition is uniform
+    for (int tile = tile0; tile * W < ncols; tile += gridDim.x) {
+    // the lane's indices are made opaque per pass, so the address arithmetic
+    // derived from them is recomputed in the pass rather than kept live
+    // across it (which would cost occupancy)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int c = tid / TPC, th = tid - c * TPC;
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem) + c * LL;
     const int kx = tile * W + c;
     const bool live = kx < ncols;
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem) + c * LL;
 
     // free-space symbol: rows_in, rows_out <= n/2, so the zero-padded upper
     // half is neither loaded nor transformed forward, nor produced inverse
@@ -253,6 +265,7 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
                 if (p < rows_out) BR[(size_t)p * pitch + kx] = r[v];
             }
         }
+    }
     }
 }
 
@@ -387,12 +400,49 @@ void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
     ASPHER_LAUNCH_CHECK();
 }
 
+// Concurrent workgroups of a kernel on the device (occupancy x CUs), cached.
+int resident_blocks(const void* fn, int block, std::size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, std::size_t>, int> done;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_tuple(fn, block, lds);
+    auto it = done.find(key);
+    if (it != done.end()) return it->second;
+    int per_cu = 0, dev = 0, cus = 0;
+    ASPHER_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds));
+    ASPHER_HIP(hipGetDevice(&dev));
+    ASPHER_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int r = std::max(1, per_cu) * std::max(1, cus);
+    done[key] = r;
+    return r;
+}
+
+// Grid of the column pass: when the tiles exceed one round of resident
+// workgroups by a small remainder (at most 1/16 of a round), the remainder
+// is folded into the first blocks as a second pass instead of a last round
+// that would run almost alone.
+int cols_grid(int tiles, int resident) {
+    if (tiles <= resident) return tiles;
+    const int rem = tiles % resident;
+    if (rem == 0 || rem > resident / 16) return tiles;
+    return tiles - rem;
+}
+
 template <typename T, int LOGN>
 void launch_cols(const RbCols<T>& a, hipStream_t s) {
     constexpr int n = 1 << LOGN;
     const int block = a.W * (n / 16);
-    const dim3 grid((a.ncols + a.W - 1) / a.W, a.bt.count);
+    const int tiles = (a.ncols + a.W - 1) / a.W;
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.W * rb::lds_len(n);
+    int gx = tiles;
+    if (a.sym != 3 && a.bt.count == 1) {
+        const void* f = a.sym == 0 ? (const void*)k_rb_cols<T, LOGN, 0>
+                        : a.sym == 1 ? (const void*)k_rb_cols<T, LOGN, 1>
+                                     : (const void*)k_rb_cols<T, LOGN, 2>;
+        allow_lds_rb(f, lds);
+        gx = cols_grid(tiles, resident_blocks(f, block, lds));
+    }
+    const dim3 grid(gx, a.bt.count);
 #define RC_ARGS a.AT, a.pitchT, a.BR, a.pitch, a.ncols, a.rows_in, a.rows_out, a.W, a.tw, a.ktab, \
         a.kp, a.sc, a.kout, a.part, a.remap, a.bt
 #define RC_CASE(S)                                                     \

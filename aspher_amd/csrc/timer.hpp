@@ -25,15 +25,22 @@ public:
         (void)hipEventRecord(e, s);
         pending_.emplace_back(cur_, e);
     }
-    // call after a stream synchronisation
+    // takes the pairs that have completed (all of them after a stream
+    // synchronisation); pairs still in flight stay pending
     void drain() {
-        for (auto& pr : pending_) {
+        std::size_t keep = 0;
+        for (std::size_t i = 0; i < pending_.size(); ++i) {
+            auto& pr = pending_[i];
+            if (hipEventQuery(pr.second) != hipSuccess) {
+                pending_[keep++] = pr;
+                continue;
+            }
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) total_ += ms * 1e-3;
             pool_.push_back(pr.first);
             pool_.push_back(pr.second);
         }
-        pending_.clear();
+        pending_.resize(keep);
     }
     double seconds() const { return total_; }
 
