@@ -321,20 +321,22 @@ k_tupdate(std::int64_t N, const T* __restrict__ g, T* __restrict__ gprev,
           const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
           T* __restrict__ t, const double* __restrict__ scal, Ens ens) {
     ENS_SETUP(N);
-    g += o_; gprev += o_; mask += o_; t += o_;
+    g += o_; mask += o_; t += o_;
+    if (gprev) gprev += o_;
     if (PC) z += o_;
     scal = ENS_SCAL(scal);
     const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
     const T beta = static_cast<T>(scal[S_BETA]);
+    // gprev == nullptr: the caller keeps g_prev by rotating buffers (no copy)
+    const bool copy = gprev != nullptr;
     ENS_LOOP(i) {
-        const T gi = g[i];
         if (mask[i]) {
-            const T zi = PC ? T(z[i] - zmean) : gi;
+            const T zi = PC ? T(z[i] - zmean) : g[i];
             t[i] = zi + beta * t[i];
         } else {
             t[i] = T(0);
         }
-        gprev[i] = gi;
+        if (copy) gprev[i] = g[i];
     }
 }
 

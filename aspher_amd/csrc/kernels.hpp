@@ -63,7 +63,7 @@ template <typename T> struct VecOps {
     static void cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
                       const std::uint8_t* mask, const double* scal, bool has_pc,
                       double* part, int nblk, hipStream_t s, const Ens& ens = Ens{});
-    // t = mask ? z + beta t : 0 ; gprev = g
+    // t = mask ? z + beta t : 0 ; gprev = g (skipped when gprev is null)
     static void tupdate(std::int64_t N, const T* g, T* gprev, const T* z,
                         const std::uint8_t* mask, T* t, const double* scal,
                         bool has_pc, hipStream_t s, const Ens& ens = Ens{});

@@ -133,6 +133,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     }
     V::fill(N, T_, 0.0, s);
     V::fill(N, GP, 0.0, s);
+    V::fill(N, G, 0.0, s);
     {
         const int idx[6] = {S_PTOT, S_GREF, S_USEPR, S_PBAR, S_GOLD, S_DELTA};
         const double val[6] = {P_total, g_ref, opt.use_pr ? 1.0 : 0.0,
@@ -276,7 +277,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         }
         V::cgdot(N, G, GP, Z, mask, scal, has_pc, part, nblk, s);
         finalize<Real>(part, nblk, 3, kS3, dg, scal, STEP_CGDOT, s);
-        V::tupdate(N, G, GP, Z, mask, T_, scal, has_pc, s);
+        V::tupdate(N, G, nullptr, Z, mask, T_, scal, has_pc, s);
         apply(T_, R);
         V::line1(N, R, G, T_, mask, part, nblk, s);
         finalize<Real>(part, nblk, 2, kS2, dl1, scal, STEP_LINE1, s);
@@ -321,10 +322,18 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     bool pending = false;
     Diag d;
 
+    // g_prev by rotation: the gradient buffer of the previous iteration
+    // becomes g_prev and the next evaluation writes the other one. It equals
+    // the reference's copy (g_prev = g in the CG step) whenever g_prev is
+    // used: beta carries the factor delta, which is 1 only after a CG step
+    // without support change, i.e. when the previous iteration took the CG
+    // branch; after a restart (delta = 0) its value does not enter. Both
+    // buffers start at zero, so it is always finite.
     int it = 0;
     for (it = 0; it < opt.max_iter; ++it) {
         if (!u_fresh) apply(P, U);
         u_fresh = false;
+        std::swap(G, GP);
         evaluate();
         if (speculate) {
             ctx.copy_scal_async();
