@@ -251,6 +251,32 @@ __device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<double>* __rest
     }
 }
 
+// exp(-2 pi i v/32), v = 0..15: the twiddles of a radix-16 stage of stride 2
+// (followed by a radix-2 stage), whose groups take either these or ones.
+constexpr double kW32re[16] = {
+    1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+    0.7071067811865475244, 0.55557023301960222474, 0.38268343236508977173,
+    0.19509032201612826785, 0.0, -0.19509032201612826785, -0.38268343236508977173,
+    -0.55557023301960222474, -0.7071067811865475244, -0.83146961230254523708,
+    -0.92387953251128675613, -0.98078528040323044913};
+constexpr double kW32im[16] = {
+    0.0, -0.19509032201612826785, -0.38268343236508977173, -0.55557023301960222474,
+    -0.7071067811865475244, -0.83146961230254523708, -0.92387953251128675613,
+    -0.98078528040323044913, -1.0, -0.98078528040323044913, -0.92387953251128675613,
+    -0.83146961230254523708, -0.7071067811865475244, -0.55557023301960222474,
+    -0.38268343236508977173, -0.19509032201612826785};
+
+template <typename T, bool INV>
+__device__ __forceinline__ void twiddle_w32(cx<T>* a, int low) {
+    if (low == 0) return;
+#pragma unroll
+    for (int v = 1; v < 16; ++v) {
+        const cx<T> w = {static_cast<T>(kW32re[v]),
+                         static_cast<T>(INV ? -kW32im[v] : kW32im[v])};
+        a[v] = a[v] * w;
+    }
+}
+
 // Stage s of the forward (INV=false, DFT then twiddle) or inverse (INV=true,
 // twiddle then DFT) transform on the registers of thread th. PR (stage 0
 // only, where every thread holds one group of 16 at positions v*S_0 + th):
@@ -266,16 +292,23 @@ __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __rest
     for (int q = 0; q < Gs; ++q) {
         cx<T>* a = r + q * Ns;
         const int low = (th * Gs + q) % Ss;
+        // stride 2: the group's twiddles are W32^v (low = 1) or ones,
+        // constants rather than table reads and products
+        constexpr bool W32 = (Ns == 16 && Ss == 2);
         if constexpr (INV) {
-            if constexpr (s < Sh::m - 1)
-                twiddle_group<T, Ns, true>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
+            if constexpr (s < Sh::m - 1) {
+                if constexpr (W32) twiddle_w32<T, true>(a, low);
+                else twiddle_group<T, Ns, true>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
+            }
             if constexpr (PR == 2) dft16_hout<T, true>(a);
             else dft<T, Ns, true>(a);
         } else {
             if constexpr (PR == 1) dft16_zin<T, false>(a);
             else dft<T, Ns, false>(a);
-            if constexpr (s < Sh::m - 1)
-                twiddle_group<T, Ns, false>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
+            if constexpr (s < Sh::m - 1) {
+                if constexpr (W32) twiddle_w32<T, false>(a, low);
+                else twiddle_group<T, Ns, false>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
+            }
         }
     }
 }
