@@ -118,6 +118,36 @@ def cpu_baseline(Ns, gap, fp64_stages, workers):
                 sample_s=t_build + 2 * t_mv + t_solve)
 
 
+def run_adjoint(hc, Ns=1024, pbar=0.005):
+    """Gradient of L = 1/2 sum (p - p_target)^2 with respect to the gap (FP64
+    forward solve to tol 1e-8, adjoint to relative residual 1e-10) at
+    Ns = 1024, free space and periodic; p_target is the solved pressure of
+    the seed-1 surface. Returns the forward and adjoint times (operators
+    cached, second call timed)."""
+    out = {}
+    g0 = hc.surfaces.rough_gap(Ns, H=0.8, seed=0)
+    gt = hc.surfaces.rough_gap(Ns, H=0.8, seed=1)
+    for backend in ("fft", "periodic"):
+        kw = dict(coarsest=64, backend=backend, tol=1e-8, precision="double",
+                  reuse_operators=True)
+        pt = hc.solve_nested(Ns, gt, pbar, **kw).pressure.ravel()
+        hc.solve_nested(Ns, g0, pbar, **kw)
+        t0 = time.perf_counter()
+        r = hc.solve_nested(Ns, g0, pbar, **kw)
+        t_fwd = time.perf_counter() - t0
+        s = hc.ContactSolver(Ns, backend=backend, shared_operator=True)
+        w = r.pressure.ravel() - pt
+        s.adjoint(r, w, tol=1e-10)
+        t0 = time.perf_counter()
+        _, _, info = s.adjoint(r, w, tol=1e-10)
+        t_adj = time.perf_counter() - t0
+        out[backend] = dict(forward_s=round(t_fwd, 4), adjoint_s=round(t_adj, 4),
+                            ratio=round(t_adj / t_fwd, 3), iterations=info["iterations"],
+                            converged=bool(info["converged"]), precond=info["precond"])
+    hc.clear_operator_cache()
+    return out
+
+
 def ensemble_problems(hc, n=64, seed=0):
     """The batched workload: sizes from {256, 512, 1024, 2048}^2, Hurst
     uniform in [0.6, 0.9], rms 0.002, |k| in [4, 64], p_nominal log-uniform in
@@ -317,6 +347,12 @@ def main():
                                                            wp, gap, sargs, res, walls), prov)
         log("[bench] ledger records appended to %s" % args.ledger)
 
+    log("[bench] adjoint (Ns=1024, gradient of 1/2 |p - p_target|^2)")
+    adj = run_adjoint(hc)
+    for k, v in adj.items():
+        log("[bench]   %s: forward %.3f s, adjoint %.3f s (x%.2f)" % (k, v["forward_s"],
+                                                                   v["adjoint_s"], v["ratio"]))
+
     ens = None
     if not args.no_ensemble:
         log("[bench] ensemble (64 instances)")
@@ -379,6 +415,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(out))
 
