@@ -292,3 +292,30 @@ def test_nested_rejects_out_of_contract(hc):
         hc.solve_nested(grid_size=128, gap=g, p_nominal=0.01, coarsest=32, backend="nope")
     with pytest.raises(ValueError):
         hc.solve_nested(grid_size=128, gap=g, p_nominal=0.0, coarsest=32, backend="fft")
+
+
+@pytest.mark.parametrize("backend,precision,tol,max_iter", [
+    ("fft", "double", 1e-10, 20000),
+    ("fft", "float", 1e-6, 20000),
+    ("periodic", "double", 1e-25, 300),     # stagnation: stall, forced identification
+    ("fft", "double", 1e-10, 7),            # budget ends right after a CG step
+])
+def test_overlapped_cg_step_matches_synchronous(hc, backend, precision, tol, max_iter):
+    """The solver enqueues each CG step before the host reads the iteration's
+    diagnostics and drops it on the other branches. An opt-in cost gate that
+    never fires (1e9) runs the same iteration with the host synchronised
+    before each step: every result must be bit-identical."""
+    Ns = 128
+    g = hc.surfaces.rough_gap(Ns, H=0.8, seed=3) if backend == "fft" else _periodic_rough(Ns)
+    kw = dict(grid_size=Ns, gap=g, p_nominal=0.01, coarsest=32, tol=tol, max_iter=max_iter,
+              backend=backend, precision=precision, allow_tolerance_relaxation=True)
+    a = hc.solve_nested(**kw)
+    b = hc.solve_nested(precond_cost_gate=1e9, **kw)
+    assert a.status == b.status and a.status_reason == b.status_reason
+    assert a.iterations == b.iterations and a.matvec_count == b.matvec_count
+    assert a.precond_count == b.precond_count
+    assert a.identification_steps == b.identification_steps
+    assert not b.precond_dropped
+    np.testing.assert_array_equal(a.pressure, b.pressure)
+    np.testing.assert_array_equal(a.gap, b.gap)
+    assert [s["iterations"] for s in a.stage_stats] == [s["iterations"] for s in b.stage_stats]
