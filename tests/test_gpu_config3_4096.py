@@ -1,0 +1,61 @@
+"""The headline workload at its real size: free-space rough contact at
+Ns = 4096 (H = 0.8, rms 0.002, |k| in [4, 64], p_nominal 0.005, tol 1e-6,
+nested from 64), mixed precision (float_then_double), FP64, and mixed with
+the active set. Each returned pressure is certified by the oracle's
+independent FP64 operator (scipy.fft of the C Love table), and the mixed
+result must match the FP64 one: contact area within 1e-4, pressure and mean
+gap close."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as O  # noqa: E402
+
+NS, PBAR, TOL = 4096, 0.005, 1e-6
+
+
+@pytest.fixture(scope="module")
+def c3():
+    import aspher_amd as hc
+    assert hc.device_count() >= 1, "no GPU visible"
+    O.WORKERS = 16
+    g = hc.surfaces.rough_gap(NS, H=0.8, k_low=4, k_high=64, rms=0.002, seed=0)
+    kw = dict(grid_size=NS, gap=g, p_nominal=PBAR, coarsest=64, tol=TOL, backend="fft",
+              reuse_operators=True)
+    out = {
+        "mixed": hc.solve_nested(precision="float_then_double", **kw),
+        "fp64": hc.solve_nested(precision="double", **kw),
+        "active": hc.solve_nested(precision="float_then_double", active_set=True, **kw),
+    }
+    hc.clear_operator_cache()
+    op = O.FreeSpaceOperator(NS)
+    return g, out, op
+
+
+@pytest.mark.parametrize("which", ["mixed", "fp64", "active"])
+def test_c3_certified_by_oracle(c3, which):
+    g, out, op = c3
+    r = out[which]
+    print(which, r, flush=True)
+    assert r.status == "converged"
+    assert r.fw_error <= TOL and r.penetration_error <= TOL
+    g_ref = max(np.ptp(g), PBAR)
+    d = O.diagnose(op.matvec, g, r.pressure.ravel(), PBAR * NS * NS, g_ref)
+    assert d["fw_error"] <= 1.01 * TOL and d["penetration"] <= 1.01 * TOL, d
+    assert d["load_error"] <= 1e-12 and d["min_pressure"] >= 0.0
+    assert abs(d["n_contact"] / NS ** 2 - r.contact_area) <= 1.0 / NS ** 2
+
+
+def test_c3_mixed_matches_fp64(c3):
+    _, out, _ = c3
+    m, d, a = out["mixed"], out["fp64"], out["active"]
+    for r in (m, a):
+        assert abs(r.contact_area - d.contact_area) <= 1e-4
+        p, q = r.pressure.ravel(), d.pressure.ravel()
+        assert np.linalg.norm(p - q) / np.linalg.norm(q) <= 1e-4
+        assert abs(r.gap.mean() - d.gap.mean()) <= 1e-6 * np.ptp(d.gap)
+    # the float identification stage precedes an FP64 polish at the full tolerance
+    st = m.stage_stats
+    assert st[-2]["precision"] == "float" and st[-1]["name"] == "polish:4096"
+    assert st[-1]["precision"] == "double" and st[-1]["status"] == "converged"
