@@ -67,6 +67,21 @@ def profiled_traffic(kernel):
     return None
 
 
+def profiled_issue(kernel):
+    """VALU instructions per wave and VALU issue fraction of `kernel` from the
+    same summary (SQ_INSTS_VALU over one wave instruction per CU per clock at
+    2.4 GHz); None when absent."""
+    path = os.path.join(ROOT, "profiles", "matvec_fft_ns4096_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| `%s` |" % kernel):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            return dict(valu_instr_per_wave=int(cells[-2]), valu_busy=float(cells[-1]),
+                        valu_peak="1 wave64 instruction per CU per clock (no packed FP32)")
+    return None
+
+
 def run_solve(hc, Ns, gptr, precision, reuse, float_floor, active=False):
     kw = dict(coarsest=64, backend="fft", precision=precision, light_result=False,
               reuse_operators=reuse, active_set=active)
@@ -331,7 +346,8 @@ def main():
                     launch_ms=round(m32["stages_ms"][k], 4),
                     matvec_gbs=round(m32["gbs"], 1),
                     matvec_frac=round(m32["gbs"] / HBM_PEAK_GBS, 4),
-                    matvec_ms=round(m32["ms"], 4))
+                    matvec_ms=round(m32["ms"], 4),
+                    issue=profiled_issue(kname) if Ns == 4096 else None)
 
     if args.ledger:
         from aspher_amd import ledger
