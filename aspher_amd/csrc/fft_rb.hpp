@@ -318,6 +318,28 @@ struct NoHook {
     __device__ __forceinline__ void operator()() const {}
 };
 
+// Positions of the registers of thread th in stage s's grouping, as
+// base(th) + off(q, v) = Shape::pos(s, th*G + q, v): a radix-16 stage holds
+// one group (off = v*S), the last stage of a radix below 16 holds G groups
+// of stride 1 (base = 16 th, off = q*N + v). The skew does not carry
+// between the two parts (base mod 16 plus off mod 16 stays below 16 in both
+// cases), so lds_ix(base + off) = lds_ix(base) + off + off/16: one address
+// per thread and compile-time offsets.
+template <int LOGN, int s>
+struct Grouping {
+    using Sh = Shape<LOGN>;
+    static constexpr int N = Sh::N(s), S = Sh::S(s), G = Sh::G(s);
+    static_assert(G == 1 || S == 1, "only the last stage holds several groups");
+    static __device__ __forceinline__ int base(int th) {
+        if constexpr (G == 1) return (th / S) * N * S + th % S;
+        else return 16 * th;
+    }
+    static constexpr int off(int q, int v) {
+        const int o = G == 1 ? v * S : q * N + v;
+        return o + (o >> 4);
+    }
+};
+
 // Regroup registers from stage sa's grouping to stage sb's through LDS.
 // `hook` runs between the LDS writes and the barrier, where the registers
 // of the data are free: a caller issues global loads there (the kernel
@@ -326,19 +348,21 @@ struct NoHook {
 template <typename T, int LOGN, int sa, int sb, typename Hook = NoHook>
 __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
                                          const Hook& hook = Hook{}) {
-    using Sh = Shape<LOGN>;
-    constexpr int Na = Sh::N(sa), Ga = Sh::G(sa), Nb = Sh::N(sb), Gb = Sh::G(sb);
+    using A = Grouping<LOGN, sa>;
+    using B = Grouping<LOGN, sb>;
     __syncthreads();
+    cx<T>* wa = lds + lds_ix(A::base(th));
 #pragma unroll
-    for (int q = 0; q < Ga; ++q)
+    for (int q = 0; q < A::G; ++q)
 #pragma unroll
-        for (int v = 0; v < Na; ++v) lds[lds_ix(Sh::pos(sa, th * Ga + q, v))] = r[q * Na + v];
+        for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v];
     hook();
     __syncthreads();
+    const cx<T>* rbp = lds + lds_ix(B::base(th));
 #pragma unroll
-    for (int q = 0; q < Gb; ++q)
+    for (int q = 0; q < B::G; ++q)
 #pragma unroll
-        for (int v = 0; v < Nb; ++v) r[q * Nb + v] = lds[lds_ix(Sh::pos(sb, th * Gb + q, v))];
+        for (int v = 0; v < B::N; ++v) r[q * B::N + v] = rbp[B::off(q, v)];
 }
 
 template <typename T, int LOGN, int s, bool ZIN, typename Hook>
