@@ -365,27 +365,63 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
         for (int v = 0; v < B::N; ++v) r[q * B::N + v] = rbp[B::off(q, v)];
 }
 
-template <typename T, int LOGN, int s, bool ZIN, typename Hook>
-__device__ __forceinline__ void forward_from(cx<T>* r, cx<T>* lds, int th,
+// The same regrouping through a float LDS buffer of half the size: real
+// parts first, then imaginary parts (four barriers instead of two). The
+// second write reads only the imaginary parts, which the first read left
+// untouched. Half the LDS per transform lets more workgroups be resident.
+template <typename T, int LOGN, int sa, int sb, typename Hook = NoHook>
+__device__ __forceinline__ void exchange(cx<T>* r, float* lds, int th,
+                                         const Hook& hook = Hook{}) {
+    static_assert(std::is_same_v<T, float>, "split exchange: FP32 data only");
+    using A = Grouping<LOGN, sa>;
+    using B = Grouping<LOGN, sb>;
+    float* wa = lds + lds_ix(A::base(th));
+    const float* rbp = lds + lds_ix(B::base(th));
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < A::G; ++q)
+#pragma unroll
+        for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v].re;
+    hook();
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B::G; ++q)
+#pragma unroll
+        for (int v = 0; v < B::N; ++v) r[q * B::N + v].re = rbp[B::off(q, v)];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < A::G; ++q)
+#pragma unroll
+        for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v].im;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B::G; ++q)
+#pragma unroll
+        for (int v = 0; v < B::N; ++v) r[q * B::N + v].im = rbp[B::off(q, v)];
+}
+
+// LDS element type L: cx<T> (one pass per exchange) or float (split passes)
+template <typename T, int LOGN, int s, bool ZIN, typename Hook, typename L>
+__device__ __forceinline__ void forward_from(cx<T>* r, L* lds, int th,
                                              const cx<double>* __restrict__ tw, int twmul,
                                              const Hook& hook) {
     stage<T, LOGN, s, false, (s == 0 && ZIN) ? 1 : 0>(r, th, tw, twmul);
     if constexpr (s + 1 < Shape<LOGN>::m) {
         if constexpr (s + 2 == Shape<LOGN>::m) exchange<T, LOGN, s, s + 1>(r, lds, th, hook);
         else exchange<T, LOGN, s, s + 1>(r, lds, th);
-        forward_from<T, LOGN, s + 1, ZIN>(r, lds, th, tw, twmul, hook);
+        forward_from<T, LOGN, s + 1, ZIN, Hook, L>(r, lds, th, tw, twmul, hook);
     } else if constexpr (s == 0) {
         hook();   // single-stage transform: no exchange to hide the loads in
     }
 }
 
-template <typename T, int LOGN, int s, bool HOUT>
-__device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
+template <typename T, int LOGN, int s, bool HOUT, typename L>
+__device__ __forceinline__ void inverse_from(cx<T>* r, L* lds, int th,
                                              const cx<double>* __restrict__ tw, int twmul) {
     stage<T, LOGN, s, true, (s == 0 && HOUT) ? 2 : 0>(r, th, tw, twmul);
     if constexpr (s > 0) {
         exchange<T, LOGN, s, s - 1>(r, lds, th);
-        inverse_from<T, LOGN, s - 1, HOUT>(r, lds, th, tw, twmul);
+        inverse_from<T, LOGN, s - 1, HOUT, L>(r, lds, th, tw, twmul);
     }
 }
 
@@ -394,16 +430,16 @@ __device__ __forceinline__ void inverse_from(cx<T>* r, cx<T>* lds, int th,
 // the input (positions >= n/2, i.e. v >= 8 of stage 0) is zero; HOUT: only
 // the lower half of the output is used.
 // `hook` runs once, inside the last exchange (see `exchange`).
-template <typename T, int LOGN, bool ZIN = false, typename Hook = NoHook>
-__device__ __forceinline__ void forward(cx<T>* r, cx<T>* lds, int th,
+template <typename T, int LOGN, bool ZIN = false, typename Hook = NoHook, typename L = cx<T>>
+__device__ __forceinline__ void forward(cx<T>* r, L* lds, int th,
                                         const cx<double>* __restrict__ tw, int twmul,
                                         const Hook& hook = Hook{}) {
-    forward_from<T, LOGN, 0, ZIN>(r, lds, th, tw, twmul, hook);
+    forward_from<T, LOGN, 0, ZIN, Hook, L>(r, lds, th, tw, twmul, hook);
 }
-template <typename T, int LOGN, bool HOUT = false>
-__device__ __forceinline__ void inverse(cx<T>* r, cx<T>* lds, int th,
+template <typename T, int LOGN, bool HOUT = false, typename L = cx<T>>
+__device__ __forceinline__ void inverse(cx<T>* r, L* lds, int th,
                                         const cx<double>* __restrict__ tw, int twmul) {
-    inverse_from<T, LOGN, Shape<LOGN>::m - 1, HOUT>(r, lds, th, tw, twmul);
+    inverse_from<T, LOGN, Shape<LOGN>::m - 1, HOUT, L>(r, lds, th, tw, twmul);
 }
 
 } // namespace rb
