@@ -26,6 +26,16 @@ struct DeviceError : std::runtime_error {
 
 #define ASPHER_LAUNCH_CHECK() ASPHER_HIP(hipGetLastError())
 
+// Device memory of the library's own buffers, with reuse: a freed block is
+// kept (per device, up to a capacity) and handed out again for a request of
+// the same rounded size, so repeated solves do not pay hipMalloc/hipFree per
+// level. dev_free synchronises the device first, as hipFree does, so a
+// reused block is never still in use by queued work.
+void* dev_alloc(std::size_t bytes);
+void dev_free(void* p, std::size_t bytes) noexcept;
+// Returns every cached block to the runtime (the device must be idle).
+void dev_release_cached() noexcept;
+
 // Interleaved complex value; alignment lets the compiler emit 8/16-byte
 // vector loads and stores for one element.
 template <typename T>

@@ -53,6 +53,109 @@ const int kSum2[2] = {RSUM, RSUM};
 
 // ── Context ────────────────────────────────────────────────────────────────
 
+// ── device memory with reuse ───────────────────────────────────────────────
+
+namespace {
+
+struct DevPool {
+    std::multimap<std::size_t, void*> blocks;   // rounded size -> cached block
+    std::size_t cached = 0, cap = 0;
+};
+
+std::mutex& pool_mutex() {
+    static std::mutex* m = new std::mutex;   // never destroyed: no exit-order issues
+    return *m;
+}
+std::map<int, DevPool>& pools() {
+    static auto* p = new std::map<int, DevPool>;
+    return *p;
+}
+std::size_t pool_round(std::size_t bytes) {
+    return (bytes + 4095) & ~static_cast<std::size_t>(4095);
+}
+
+void evict_all(DevPool& P) {
+    for (auto& kv : P.blocks) (void)hipFree(kv.second);
+    P.blocks.clear();
+    P.cached = 0;
+}
+
+} // namespace
+
+void* dev_alloc(std::size_t bytes) {
+    if (bytes == 0) return nullptr;
+    const std::size_t sz = pool_round(bytes);
+    int dev = 0;
+    ASPHER_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lk(pool_mutex());
+        DevPool& P = pools()[dev];
+        auto it = P.blocks.find(sz);
+        if (it != P.blocks.end()) {
+            void* q = it->second;
+            P.blocks.erase(it);
+            P.cached -= sz;
+            return q;
+        }
+    }
+    void* q = nullptr;
+    if (hipMalloc(&q, sz) != hipSuccess) {
+        // out of memory: give the cached blocks back and try once more
+        (void)hipGetLastError();
+        (void)hipDeviceSynchronize();
+        {
+            std::lock_guard<std::mutex> lk(pool_mutex());
+            evict_all(pools()[dev]);
+        }
+        ASPHER_HIP(hipMalloc(&q, sz));
+    }
+    return q;
+}
+
+void dev_free(void* q, std::size_t bytes) noexcept {
+    if (!q) return;
+    const std::size_t sz = pool_round(bytes);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(q);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    DevPool& P = pools()[dev];
+    if (P.cap == 0) {
+        std::size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            (void)hipGetLastError();
+            tot = std::size_t(16) << 30;
+        }
+        P.cap = std::min<std::size_t>(tot / 8, std::size_t(24) << 30);
+    }
+    if (sz > P.cap) {
+        (void)hipFree(q);
+        return;
+    }
+    P.blocks.emplace(sz, q);
+    P.cached += sz;
+    while (P.cached > P.cap && !P.blocks.empty()) {
+        auto it = std::prev(P.blocks.end());   // largest first
+        (void)hipFree(it->second);
+        P.cached -= it->first;
+        P.blocks.erase(it);
+    }
+}
+
+void dev_release_cached() noexcept {
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    for (auto& kv : pools()) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(kv.first);
+        evict_all(kv.second);
+        (void)hipSetDevice(cur);
+    }
+}
+
 Context::Context() {
     ASPHER_HIP(hipStreamCreate(&stream));
     scal.alloc(S_COUNT);

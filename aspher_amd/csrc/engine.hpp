@@ -31,11 +31,11 @@ struct DevBuf {
     }
     void alloc(std::size_t count) {
         reset();
-        if (count) ASPHER_HIP(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count));
+        if (count) p = static_cast<T*>(dev_alloc(sizeof(T) * count));
         n = count;
     }
     void reset() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p, sizeof(T) * n);
         p = nullptr;
         n = 0;
     }

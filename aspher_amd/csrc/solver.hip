@@ -531,8 +531,14 @@ std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L,
 }
 
 void clear_operator_cache() {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    g_cache.clear();
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache.clear();
+    }
+    // the freed operators' memory, and every other cached block, goes back
+    // to the runtime
+    (void)hipDeviceSynchronize();
+    dev_release_cached();
 }
 
 void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double p_bar,

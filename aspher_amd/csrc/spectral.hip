@@ -339,13 +339,13 @@ Twiddles<T>::Twiddles(int n_) : n(n_) {
         if (a == 0) { c = 1; s = 0; }
         h[t] = {static_cast<T>(c), static_cast<T>(-s)};
     }
-    ASPHER_HIP(hipMalloc(&dev, sizeof(cx<T>) * n));
+    dev = static_cast<cx<T>*>(dev_alloc(sizeof(cx<T>) * n));
     ASPHER_HIP(hipMemcpy(dev, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
 }
 
 template <typename T>
 Twiddles<T>::~Twiddles() {
-    if (dev) (void)hipFree(dev);
+    dev_free(dev, sizeof(cx<T>) * n);
 }
 
 template <typename T>
@@ -383,8 +383,10 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
         a_rows_ = static_cast<std::size_t>(std::max(rows_out, 1));
         br_stride_ = (std::int64_t)a_rows_ * pitch;
         at_stride_ = (std::int64_t)nh * pitchT_;
-        ASPHER_HIP(hipMalloc(&A_, cs * (std::size_t)br_stride_ * batch_));
-        ASPHER_HIP(hipMalloc(&AT_, cs * (std::size_t)at_stride_ * batch_));
+        a_bytes_ = cs * (std::size_t)br_stride_ * batch_;
+        at_bytes_ = cs * (std::size_t)at_stride_ * batch_;
+        A_ = static_cast<cx<T>*>(dev_alloc(a_bytes_));
+        AT_ = static_cast<cx<T>*>(dev_alloc(at_bytes_));
         return;
     }
     if (batch_ != 1)
@@ -411,13 +413,14 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
     }
     tw_ = std::make_unique<Twiddles<T>>(n);
     a_rows_ = static_cast<std::size_t>(std::max(rows_in, rows_out));
-    ASPHER_HIP(hipMalloc(&A_, cs * a_rows_ * pitch));
+    a_bytes_ = cs * a_rows_ * pitch;
+    A_ = static_cast<cx<T>*>(dev_alloc(a_bytes_));
 }
 
 template <typename T>
 SpectralPlan<T>::~SpectralPlan() {
-    if (A_) (void)hipFree(A_);
-    if (AT_) (void)hipFree(AT_);
+    dev_free(A_, a_bytes_);
+    dev_free(AT_, at_bytes_);
 }
 
 template <typename T>

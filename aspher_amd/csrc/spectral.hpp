@@ -146,6 +146,7 @@ private:
                   double* part, hipStream_t s);
 
     std::unique_ptr<Twiddles<T>> tw_;
+    std::size_t a_bytes_ = 0, at_bytes_ = 0;
     cx<T>* A_ = nullptr;          // row-major half spectrum (BR of the rb path)
     std::size_t a_rows_ = 0;
     bool pow2_ = false;
