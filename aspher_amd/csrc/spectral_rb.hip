@@ -315,13 +315,17 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
     }
     __syncthreads();
     // Z_k = (X_k + conj X_{L-k}) + i (X_k - conj X_{L-k}) exp(+2 pi i k/n),
-    // gathered straight into the digit-reversed grouping of the inverse
+    // gathered straight into the digit-reversed grouping of the inverse.
+    // The last digit v of a group's positions is the top digit of k:
+    // k = kg + v L/Nl, so W^k = W^kg W_32^(16 v/Nl) (one table read per group).
     cx<T> r[16];
 #pragma unroll
-    for (int q = 0; q < Gl; ++q)
+    for (int q = 0; q < Gl; ++q) {
+        const int kg = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, 0));
+        const cx<double> wg = tw[kg];
 #pragma unroll
         for (int v = 0; v < Nl; ++v) {
-            const int k = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+            const int k = kg + v * (L / Nl);
             cx<T> Xk = lds[rb::lds_ix(k)], Xc = lds[rb::lds_ix(L - k)];
             if (k == 0) {
                 Xk.im = T(0);
@@ -329,10 +333,13 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
             }
             Xc = conjg(Xc);
             const cx<T> S = Xk + Xc;
-            const cx<double> wd = tw[k & ~63] * tw[k & 63];
+            constexpr int js = 16 / Nl;
+            const cx<double> wd =
+                v == 0 ? wg : wg * cx<double>{rb::kW32re[(v * js) & 15], rb::kW32im[(v * js) & 15]};
             const cx<T> D = (Xk - Xc) * cx<T>{static_cast<T>(wd.re), static_cast<T>(-wd.im)};
             r[q * Nl + v] = {S.re - D.im, S.im + D.re};
         }
+    }
     rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
     double acc = 0.0;
     if (row < nrows) {
