@@ -365,18 +365,17 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
         for (int v = 0; v < B::N; ++v) r[q * B::N + v] = rbp[B::off(q, v)];
 }
 
-// The same regrouping through a float LDS buffer of half the size: real
+// The same regrouping through a real (T) LDS buffer of half the size: real
 // parts first, then imaginary parts (four barriers instead of two). The
 // second write reads only the imaginary parts, which the first read left
 // untouched. Half the LDS per transform lets more workgroups be resident.
 template <typename T, int LOGN, int sa, int sb, typename Hook = NoHook>
-__device__ __forceinline__ void exchange(cx<T>* r, float* lds, int th,
+__device__ __forceinline__ void exchange(cx<T>* r, T* lds, int th,
                                          const Hook& hook = Hook{}) {
-    static_assert(std::is_same_v<T, float>, "split exchange: FP32 data only");
     using A = Grouping<LOGN, sa>;
     using B = Grouping<LOGN, sb>;
-    float* wa = lds + lds_ix(A::base(th));
-    const float* rbp = lds + lds_ix(B::base(th));
+    T* wa = lds + lds_ix(A::base(th));
+    const T* rbp = lds + lds_ix(B::base(th));
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < A::G; ++q)
@@ -400,7 +399,7 @@ __device__ __forceinline__ void exchange(cx<T>* r, float* lds, int th,
         for (int v = 0; v < B::N; ++v) r[q * B::N + v].im = rbp[B::off(q, v)];
 }
 
-// LDS element type L: cx<T> (one pass per exchange) or float (split passes)
+// LDS element type L: cx<T> (one pass per exchange) or T (split passes)
 template <typename T, int LOGN, int s, bool ZIN, typename Hook, typename L>
 __device__ __forceinline__ void forward_from(cx<T>* r, L* lds, int th,
                                              const cx<double>* __restrict__ tw, int twmul,
