@@ -284,10 +284,7 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
 // ── rows: half spectrum (row-major) -> real ────────────────────────────────
 // H: only the reals y[0 .. L) of n = 2L are wanted (free-space rows), so the
 // last radix-16 stage computes half of its outputs.
-// SP: the load, the split and the exchanges go through a real (T) LDS view,
-// real parts then imaginary parts, in half the LDS (more resident
-// workgroups).
-template <typename T, int LOGL, int OUT, bool H, bool SP>
+template <typename T, int LOGL, int OUT, bool H>
 __global__ void __launch_bounds__(512)
 k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
               const cx<double>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
@@ -313,116 +310,52 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
     const int row = blk * B + b;
     cx<T>* lds = lds_all + b * LL;
     cx<T> r[16];
-    if constexpr (!SP) {
-        // coalesced load of the L+1 half-spectrum values of each row
-        for (int e = threadIdx.x; e < B * (L + 1); e += blockDim.x) {
-            const int bb = e / (L + 1), k = e - bb * (L + 1);
-            const int rr = blk * B + bb;
-            lds_all[bb * LL + rb::lds_ix(k)] =
-                rr < nrows ? BR[(size_t)rr * pitch + k] : cx<T>{T(0), T(0)};
-        }
-        __syncthreads();
-        // Z_k = (X_k + conj X_{L-k}) + i (X_k - conj X_{L-k}) exp(+2 pi i k/n),
-        // gathered straight into the digit-reversed grouping of the inverse.
-        // The last digit v of a group's positions is the top digit of k:
-        // k = kg + v L/Nl, so W^k = W^kg W_32^(16 v/Nl) (one table read per group).
+    // coalesced load of the L+1 half-spectrum values of each row
+    // (e = threadIdx.x + j*blockDim.x, blockDim.x = B L/16): every load
+    // is issued before the first is waited for
+    constexpr int J = 17;
+    const int lim = B * (L + 1);
+    cx<T> xv[J];
 #pragma unroll
-        for (int q = 0; q < Gl; ++q) {
-            const int kg = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, 0));
-            const cx<double> wg = tw[kg];
+    for (int j = 0; j < J; ++j) {
+        const int e = threadIdx.x + j * (int)blockDim.x;
+        const int bb = e / (L + 1), k = e - bb * (L + 1);
+        const int rr = blk * B + bb;
+        xv[j] = (e < lim && rr < nrows) ? BR[(size_t)rr * pitch + k] : cx<T>{T(0), T(0)};
+    }
 #pragma unroll
-            for (int v = 0; v < Nl; ++v) {
-                const int k = kg + v * (L / Nl);
-                cx<T> Xk = lds[rb::lds_ix(k)], Xc = lds[rb::lds_ix(L - k)];
-                if (k == 0) {
-                    Xk.im = T(0);
-                    Xc.im = T(0);
-                }
-                Xc = conjg(Xc);
-                const cx<T> S = Xk + Xc;
-                constexpr int js = 16 / Nl;
-                const cx<double> wd =
-                    v == 0 ? wg : wg * cx<double>{rb::kW32re[(v * js) & 15], rb::kW32im[(v * js) & 15]};
-                const cx<T> D = (Xk - Xc) * cx<T>{static_cast<T>(wd.re), static_cast<T>(-wd.im)};
-                r[q * Nl + v] = {S.re - D.im, S.im + D.re};
+    for (int j = 0; j < J; ++j) {
+        const int e = threadIdx.x + j * (int)blockDim.x;
+        const int bb = e / (L + 1), k = e - bb * (L + 1);
+        if (e < lim) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+    }
+    __syncthreads();
+    // Z_k = (X_k + conj X_{L-k}) + i (X_k - conj X_{L-k}) exp(+2 pi i k/n),
+    // gathered straight into the digit-reversed grouping of the inverse.
+    // The last digit v of a group's positions is the top digit of k:
+    // k = kg + v L/Nl, so W^k = W^kg W_32^(16 v/Nl) (one table read per group).
+#pragma unroll
+    for (int q = 0; q < Gl; ++q) {
+        const int kg = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, 0));
+        const cx<double> wg = tw[kg];
+#pragma unroll
+        for (int v = 0; v < Nl; ++v) {
+            const int k = kg + v * (L / Nl);
+            cx<T> Xk = lds[rb::lds_ix(k)], Xc = lds[rb::lds_ix(L - k)];
+            if (k == 0) {
+                Xk.im = T(0);
+                Xc.im = T(0);
             }
-        }
-        rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
-    } else {
-        T* ldt = reinterpret_cast<T*>(smem);
-        T* lt = ldt + b * LL;
-        // coalesced load (e = threadIdx.x + j*blockDim.x): real parts to
-        // LDS, imaginary parts kept for the second pass
-        constexpr int J = 17;
-        const int lim = B * (L + 1);
-        T xim[J];
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int e = threadIdx.x + j * (int)blockDim.x;
-            xim[j] = T(0);
-            if (e < lim) {
-                const int bb = e / (L + 1), k = e - bb * (L + 1);
-                const int rr = blk * B + bb;
-                const cx<T> v = rr < nrows ? BR[(size_t)rr * pitch + k] : cx<T>{T(0), T(0)};
-                ldt[bb * LL + rb::lds_ix(k)] = v.re;
-                xim[j] = v.im;
-            }
-        }
-        // with X_k = p + iq, X_{L-k} = s + it, a = p - s, b = q + t and
-        // W^k = wr + i wi (q = t = 0 at k = 0):
-        //   Z_k = (p + s + a wi - b wr) + i (q - t + a wr + b wi)
-        // (the split of the non-SP path, rearranged by parts)
-        auto wk = [&](int q, int v) -> cx<T> {
-            const int kg = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, 0));
-            const cx<double> wg = tw[kg];
+            Xc = conjg(Xc);
+            const cx<T> S = Xk + Xc;
             constexpr int js = 16 / Nl;
             const cx<double> wd =
                 v == 0 ? wg : wg * cx<double>{rb::kW32re[(v * js) & 15], rb::kW32im[(v * js) & 15]};
-            return {static_cast<T>(wd.re), static_cast<T>(wd.im)};
-        };
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Gl; ++q) {
-            const int kg = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, 0));
-#pragma unroll
-            for (int v = 0; v < Nl; ++v) {
-                const int k = kg + v * (L / Nl);
-                const T p = lt[rb::lds_ix(k)], sv = lt[rb::lds_ix(L - k)];
-                const cx<T> w = wk(q, v);
-                const T a = p - sv;
-                r[q * Nl + v] = {p + sv + a * w.im, a * w.re};
-            }
+            const cx<T> D = (Xk - Xc) * cx<T>{static_cast<T>(wd.re), static_cast<T>(-wd.im)};
+            r[q * Nl + v] = {S.re - D.im, S.im + D.re};
         }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int e = threadIdx.x + j * (int)blockDim.x;
-            if (e < lim) {
-                const int bb = e / (L + 1), k = e - bb * (L + 1);
-                ldt[bb * LL + rb::lds_ix(k)] = xim[j];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Gl; ++q) {
-            const int kg = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, 0));
-#pragma unroll
-            for (int v = 0; v < Nl; ++v) {
-                const int k = kg + v * (L / Nl);
-                T qv = lt[rb::lds_ix(k)], tv = lt[rb::lds_ix(L - k)];
-                if (k == 0) {
-                    qv = T(0);
-                    tv = T(0);
-                }
-                const cx<T> w = wk(q, v);
-                const T bv = qv + tv;
-                cx<T>& z = r[q * Nl + v];
-                z.re = z.re - bv * w.re;
-                z.im = z.im + ((qv - tv) + bv * w.im);
-            }
-        }
-        rb::inverse<T, LOGL, H>(r, lt, th, tw, 2);
     }
+    rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
     double acc = 0.0;
     if (row < nrows) {
 #pragma unroll
@@ -569,27 +502,23 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
 template <typename T, int LOGL>
 void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
     constexpr int L = 1 << LOGL;
-    // FP64 rows of 2^12: the real LDS view lets 3 instead of 2 workgroups be
-    // resident (measured 14% faster at 4096^2; at 2048^2, where registers
-    // keep the residency unchanged, it was 5% slower)
-    constexpr bool SP = std::is_same_v<T, double> && LOGL == 12;
     const int block = a.B * (L / 16);
     const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
-    const std::size_t lds = (SP ? sizeof(T) : sizeof(cx<T>)) * (std::size_t)a.B * rb::lds_len(L);
+    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
 #define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap, \
         a.bt
     if (a.mode == 0 && a.ncols <= L) {
-        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, true, SP>;
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, true>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_inv<T, LOGL, 0, true, SP><<<grid, block, lds, s>>>(RI_ARGS);
+        k_rb_rows_inv<T, LOGL, 0, true><<<grid, block, lds, s>>>(RI_ARGS);
     } else if (a.mode == 0) {
-        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, false, SP>;
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, false>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_inv<T, LOGL, 0, false, SP><<<grid, block, lds, s>>>(RI_ARGS);
+        k_rb_rows_inv<T, LOGL, 0, false><<<grid, block, lds, s>>>(RI_ARGS);
     } else {
-        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 1, false, SP>;
+        const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 1, false>;
         allow_lds_rb(fn, lds);
-        k_rb_rows_inv<T, LOGL, 1, false, SP><<<grid, block, lds, s>>>(RI_ARGS);
+        k_rb_rows_inv<T, LOGL, 1, false><<<grid, block, lds, s>>>(RI_ARGS);
     }
 #undef RI_ARGS
     ASPHER_LAUNCH_CHECK();
