@@ -28,21 +28,30 @@ __device__ __forceinline__ double red_identity() {
     else return -__builtin_huge_val();
 }
 
+// Result valid in lane 0. nlanes: the wave's launched lanes (a block whose
+// size is not a multiple of 64 ends in a partial wave); a shuffle from a lane
+// at or beyond nlanes reads a register no thread wrote and is skipped. For a
+// full wave the tree and its order are the plain one.
 template <int OP>
-__device__ __forceinline__ double wave_reduce(double v) {
+__device__ __forceinline__ double wave_reduce(double v, int nlanes = 64) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = red_apply<OP>(v, __shfl_down(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(v, off, 64);
+        if (lane + off < nlanes) v = red_apply<OP>(v, o);
+    }
     return v;
 }
 
 // Reduces one value per thread over the block; the result is valid in
-// thread 0. `scratch` must hold blockDim.x/64 doubles. Contains barriers:
-// every thread of the block must call it.
+// thread 0. `scratch` must hold (blockDim.x + 63)/64 doubles. Contains
+// barriers: every thread of the block must call it.
 template <int OP>
 __device__ __forceinline__ double block_reduce(double v, double* scratch) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
-    v = wave_reduce<OP>(v);
+    const int rest = (int)blockDim.x - 64 * wave;
+    v = wave_reduce<OP>(v, rest < 64 ? rest : 64);
     __syncthreads();
     if (lane == 0) scratch[wave] = v;
     __syncthreads();

@@ -24,6 +24,16 @@ namespace aspher {
 
 namespace {
 
+// two consecutive reals as one 8- (FP32) or 16-byte (FP64) access
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair {
+    T a, b;
+};
+
+__host__ __device__ __forceinline__ bool pair_aligned(const void* p, std::size_t elt) {
+    return (reinterpret_cast<std::uintptr_t>(p) % (2 * elt)) == 0;
+}
+
 __device__ __forceinline__ int xcd_block(int b, int G) {
     const int q = G >> 3, r = G & 7;
     const int x = b & 7;
@@ -63,7 +73,7 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
 
     auto load = [&](int col) -> T {
         if (row >= nrows || col >= ncols) return T(0);
-        if constexpr (IN >= 2) {
+        if constexpr (IN >= 2 && IN <= 4) {
             // wrap-around embedding of an offset table (index d or n + d;
             // row/column Ns zero): 2 even, 3 even transposed, 4 odd-odd
             const int Ns = Ns_embed;
@@ -88,11 +98,27 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
     };
 
     cx<T> r[16];
+    // plain input on an even pitch and width from a pair-aligned base: both
+    // reals of a packed element in one access
+    const bool pairs =
+        IN == 0 && ((x_pitch | ncols) & 1) == 0 && pair_aligned(x, sizeof(T));
+    if (pairs) {
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int p = Sh::pos(0, th, v);   // packed index j: reals 2j, 2j+1
-        if (Z && v >= 8) r[v] = {T(0), T(0)};
-        else r[v] = {load(2 * p), load(2 * p + 1)};
+        for (int v = 0; v < 16; ++v) {
+            const int p = Sh::pos(0, th, v);
+            r[v] = {T(0), T(0)};
+            if ((Z && v >= 8) || row >= nrows || 2 * p >= ncols) continue;
+            const Pair<T> q =
+                *reinterpret_cast<const Pair<T>*>(x + (size_t)row * x_pitch + 2 * p);
+            r[v] = {q.a, q.b};
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int p = Sh::pos(0, th, v);   // packed index j: reals 2j, 2j+1
+            if (Z && v >= 8) r[v] = {T(0), T(0)};
+            else r[v] = {load(2 * p), load(2 * p + 1)};
+        }
     }
     rb::forward<T, LOGL, Z>(r, lds, th, tw, 2);
     // digit-reversed registers -> natural order in LDS, then split the
@@ -357,7 +383,18 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
     }
     rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
     double acc = 0.0;
-    if (row < nrows) {
+    const bool pairs =
+        OUT == 0 && ((y_pitch | ncols) & 1) == 0 && pair_aligned(y, sizeof(T));
+    if (row < nrows && pairs) {
+        // plain output on an even pitch and width: pair stores
+#pragma unroll
+        for (int v = 0; v < (H ? 8 : 16); ++v) {
+            const int p = Sh::pos(0, th, v);
+            if (2 * p >= ncols) continue;
+            *reinterpret_cast<Pair<T>*>(y + (size_t)row * y_pitch + 2 * p) =
+                Pair<T>{r[v].re, r[v].im};
+        }
+    } else if (row < nrows) {
 #pragma unroll
         for (int v = 0; v < (H ? 8 : 16); ++v) {
             const int p = Sh::pos(0, th, v);
@@ -396,6 +433,12 @@ void allow_lds_rb(const void* fn, std::size_t bytes) {
 template <typename T, int LOGL>
 void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
     constexpr int L = 1 << LOGL;
+    // every operand the mode reads must be present
+    const bool ok = a.mode == 0   ? a.x != nullptr
+                    : a.mode == 1 ? (a.x && a.mask && a.scal)
+                    : a.mode <= 4 ? a.table != nullptr
+                                  : false;
+    if (!ok) throw std::invalid_argument("row transform: missing operand for its input mode");
     const int block = a.B * (L / 16);
     const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
@@ -502,6 +545,8 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
 template <typename T, int LOGL>
 void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
     constexpr int L = 1 << LOGL;
+    const bool ok = a.y && (a.mode == 0 ? true : a.mode == 1 ? (a.mask && a.part) : false);
+    if (!ok) throw std::invalid_argument("row transform: missing operand for its output mode");
     const int block = a.B * (L / 16);
     const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
