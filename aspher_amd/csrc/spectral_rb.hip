@@ -214,7 +214,7 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
         AT += li * bt.at_stride;
         BR += li * bt.br_stride;
     }
-    constexpr int n = Sh::n, TPC = Sh::threads;
+    constexpr int n = Sh::n;
     constexpr int LL = rb::lds_len(n);
     constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
     extern __shared__ __align__(16) unsigned char smem[];
@@ -229,7 +229,10 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
     // across it (which would cost occupancy)
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    const int c = tid / TPC, th = tid - c * TPC;
+    // columns interleaved across lanes: the W columns' outputs of one row
+    // are adjacent within each store instruction (W-fold longer runs of
+    // the strided row-major stores; -14% at n = 4096, FP32)
+    const int c = tid % W, th = tid / W;
     // table symbol: the exchanges pass real and imaginary parts separately
     // through half the LDS (more resident workgroups)
     using LT = std::conditional_t<kColsSplit<T, LOGN, SYM>, T, cx<T>>;
