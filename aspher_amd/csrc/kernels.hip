@@ -810,7 +810,10 @@ struct FinSpec {
     int dst[kPartVals];
 };
 
-template <typename T>
+// NV: the number of values (spec.nv) as a compile-time constant, so every
+// load of a batch is unconditional and the whole batch is in flight at once
+// (a runtime count left one memory round trip per value)
+template <typename T, int NV>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __restrict__ scal,
            int step, Ens ens) {
@@ -824,7 +827,7 @@ k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __re
     // one pass over the partial rows for all values, then one combination
     // of the waves: per value the same order as a reduction of that value
     // alone (thread-strided rows, shuffle tree, waves in index order)
-    const int nv = spec.nv;
+    constexpr int nv = NV;
     double acc[kPartVals];
 #pragma unroll
     for (int v = 0; v < kPartVals; ++v)
@@ -1374,8 +1377,16 @@ void finalize(const double* part, int nblk, int nv, const int* ops, const int* d
         f.ops[v] = ops[v];
         f.dst[v] = dst[v];
     }
-    k_finalize<T><<<std::max(ens.ninst, 1), 256, 0, s>>>(part, nblk, f, scal,
-                                                         static_cast<int>(step), ens);
+    const dim3 grid(std::max(ens.ninst, 1));
+    const int st = static_cast<int>(step);
+    switch (nv) {
+#define FIN_CASE(V) \
+        case V: k_finalize<T, V><<<grid, 256, 0, s>>>(part, nblk, f, scal, st, ens); break;
+        FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4)
+        FIN_CASE(5) FIN_CASE(6) FIN_CASE(7) FIN_CASE(8)
+#undef FIN_CASE
+        default: throw std::invalid_argument("finalize: 1 to 8 values");
+    }
     ASPHER_LAUNCH_CHECK();
 }
 template void finalize<float>(const double*, int, int, const int*, const int*, double*,
