@@ -51,8 +51,11 @@ private:
             pool_.pop_back();
             return e;
         }
+        // elapsed times only: no system-scope fence at record (the host
+        // never reads device memory through these events), which would
+        // stall the stream for several microseconds per record
         hipEvent_t e;
-        ASPHER_HIP(hipEventCreate(&e));
+        ASPHER_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         return e;
     }
     std::vector<hipEvent_t> pool_;
