@@ -322,13 +322,19 @@ std::vector<int> factorize(int L) {
     return r;
 }
 
+// exp(-2 pi i t/n), t < n, in precision T: evaluated once per (n, T) in
+// long double (n long-double sin/cos cost ~1.5 ms at n = 8192, and every
+// operator build creates several plans) and kept for the process
 template <typename T>
-Twiddles<T>::Twiddles(int n_) : n(n_) {
+const std::vector<cx<T>>& twiddle_host(int n) {
+    static std::mutex mu;
+    static std::map<int, std::vector<cx<T>>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
     std::vector<cx<T>> h(n);
     const long double two_pi = 6.283185307179586476925286766559005768L;
     for (int t = 0; t < n; ++t) {
-        // reduce to the first octant so both components come from a small
-        // argument (symmetric to the last bit)
         long long a = t, N = n;
         // angle = 2 pi a / N ; use exact symmetries where they apply
         long double ang = two_pi * (long double)a / (long double)N;
@@ -339,6 +345,12 @@ Twiddles<T>::Twiddles(int n_) : n(n_) {
         if (a == 0) { c = 1; s = 0; }
         h[t] = {static_cast<T>(c), static_cast<T>(-s)};
     }
+    return cache.emplace(n, std::move(h)).first->second;
+}
+
+template <typename T>
+Twiddles<T>::Twiddles(int n_) : n(n_) {
+    const std::vector<cx<T>>& h = twiddle_host<T>(n);
     dev = static_cast<cx<T>*>(dev_alloc(sizeof(cx<T>) * n));
     ASPHER_HIP(hipMemcpy(dev, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
 }
