@@ -324,10 +324,10 @@ const T* Operator::table_for(const SpectralPlan<T>& pl, hipStream_t s, int& pitc
     if (pl.n != 2 * Ns_ || pl.nh != Ns_ + 1)
         throw std::invalid_argument("table_for: plan does not match the operator");
     if (!perm->p) {
-        perm->alloc(pl.table_elems());
-        pl.permute_table(kh, Ns_ + 1, perm->p, s);
+        perm->alloc(pl.table_elems_half());
+        pl.permute_table_half(kh, Ns_ + 1, perm->p, s);
     }
-    pitch = pl.n;
+    pitch = pl.table_pitch_half();
     return perm->p;
 }
 template const double* Operator::table_for<double>(const SpectralPlan<double>&, hipStream_t,

@@ -360,6 +360,66 @@ Twiddles<T>::~Twiddles() {
     dev_free(dev, sizeof(cx<T>) * n);
 }
 
+// Mirror map of the register-blocked column transform of length n = 2^logn
+// (host restatement of rb::Shape): for each column thread t, the chunk of the
+// half-size symbol table holding its 16 values and whether it reads them
+// reversed. Thread t' is t's mirror when it holds the frequencies n - ky of
+// t's registers i at registers 15 - i; a pair shares one chunk (the lower
+// thread reads it forward), every other thread gets a chunk of its own.
+// Evaluated once per size for the process.
+const std::vector<int>& mirror_map(int logn, int& nchunks) {
+    static std::mutex mu;
+    static std::map<int, std::pair<std::vector<int>, int>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(logn);
+    if (it == cache.end()) {
+        const int n = 1 << logn, A = logn / 4, RL = logn % 4, m = A + (RL ? 1 : 0);
+        int N[8], S[8], P[8];
+        for (int j = 0; j < m; ++j) N[j] = j < A ? 16 : (1 << RL);
+        for (int j = 0; j < m; ++j) {
+            S[j] = 1;
+            for (int i = j + 1; i < m; ++i) S[j] *= N[i];
+            P[j] = 1;
+            for (int i = 0; i < j; ++i) P[j] *= N[i];
+        }
+        auto pos = [&](int st, int g, int v) {
+            return (g / S[st]) * N[st] * S[st] + v * S[st] + g % S[st];
+        };
+        auto kidx = [&](int p) {
+            int k = 0;
+            for (int j = 0; j < m; ++j) k += ((p / S[j]) % N[j]) * P[j];
+            return k;
+        };
+        const int Tn = n / 16, Nl = N[m - 1], Gl = 16 / Nl;
+        std::vector<int> ky((std::size_t)Tn * 16), own_t(n), own_i(n);
+        for (int t = 0; t < Tn; ++t)
+            for (int q = 0; q < Gl; ++q)
+                for (int v = 0; v < Nl; ++v) {
+                    const int i = q * Nl + v, k = kidx(pos(m - 1, t * Gl + q, v));
+                    ky[(std::size_t)t * 16 + i] = k;
+                    own_t[k] = t;
+                    own_i[k] = i;
+                }
+        std::vector<int> km(Tn, -1);
+        int nc = 0;
+        for (int t = 0; t < Tn; ++t) {
+            if (km[t] >= 0) continue;
+            const int tp = own_t[(n - ky[(std::size_t)t * 16]) % n];
+            bool pair = tp > t && km[tp] < 0;
+            for (int i = 0; i < 16 && pair; ++i) {
+                const int k2 = (n - ky[(std::size_t)t * 16 + i]) % n;
+                pair = own_t[k2] == tp && own_i[k2] == 15 - i;
+            }
+            km[t] = nc << 1;
+            if (pair) km[tp] = (nc << 1) | 1;
+            ++nc;
+        }
+        it = cache.emplace(logn, std::make_pair(std::move(km), nc)).first;
+    }
+    nchunks = it->second.second;
+    return it->second.first;
+}
+
 template <typename T>
 SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
                               SpectralTuning tune, int batch)
@@ -399,6 +459,9 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
         at_bytes_ = cs * (std::size_t)at_stride_ * batch_;
         A_ = static_cast<cx<T>*>(dev_alloc(a_bytes_));
         AT_ = static_cast<cx<T>*>(dev_alloc(at_bytes_));
+        const std::vector<int>& km = mirror_map(logn_, kmap_chunks_);
+        kmap_ = static_cast<int*>(dev_alloc(sizeof(int) * km.size()));
+        ASPHER_HIP(hipMemcpy(kmap_, km.data(), sizeof(int) * km.size(), hipMemcpyHostToDevice));
         return;
     }
     if (batch_ != 1)
@@ -433,6 +496,7 @@ template <typename T>
 SpectralPlan<T>::~SpectralPlan() {
     dev_free(A_, a_bytes_);
     dev_free(AT_, at_bytes_);
+    if (kmap_) dev_free(kmap_, sizeof(int) * (std::size_t)(n / 16));
 }
 
 template <typename T>
@@ -483,6 +547,13 @@ void SpectralPlan<T>::permute_table(const T* kunique, int kp, T* kout, hipStream
                                     bool odd) const {
     if (!rb_) throw std::logic_error("permute_table: generic plans read the unique table");
     rb_permute_table<T>(logn_, kunique, kp, nh, kout, s, odd);
+}
+
+template <typename T>
+void SpectralPlan<T>::permute_table_half(const T* kunique, int kp, T* kout,
+                                         hipStream_t s) const {
+    if (!rb_) throw std::logic_error("permute_table_half: register-blocked plans only");
+    rb_permute_table_half<T>(logn_, kunique, kp, nh, kmap_, table_pitch_half(), kout, s);
 }
 
 template <typename T>
@@ -649,6 +720,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         c.sym = static_cast<int>(sym);
         c.ktab = ktab;
         c.kp = ktab_pitch;
+        c.kmap = kmap_;
         c.sc = sym_scale;
         c.remap = f.remap;
         RbRowsInv<T> r;

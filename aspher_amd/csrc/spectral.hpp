@@ -128,6 +128,15 @@ public:
     // of n values, pitch n); the generic path the unique (n/2+1)-wide table.
     bool table_permuted() const { return rb_; }
     std::size_t table_elems() const { return (std::size_t)nh * n; }
+    // The layout the table-symbol column pass reads (register-blocked plans):
+    // the spectrum is even in ky and the threads' frequency sets come in
+    // mirror pairs (thread t' holds n - ky of thread t, in reversed register
+    // order), so each pair shares one 16-value chunk; chunks per column and
+    // the table size follow.
+    int table_chunks() const { return kmap_chunks_; }
+    int table_pitch_half() const { return 16 * kmap_chunks_; }
+    std::size_t table_elems_half() const { return (std::size_t)nh * 16 * kmap_chunks_; }
+    void permute_table_half(const T* kunique, int kp, T* kout, hipStream_t s) const;
     void permute_table(const T* kunique, int kp, T* kout, hipStream_t s,
                        bool odd = false) const;
     SpectralTuning tuning() const { return {row_batch_, col_width_, xcd_remap_}; }
@@ -161,6 +170,9 @@ private:
     std::int64_t at_stride_ = 0, br_stride_ = 0;
     int row_batch_ = 1, col_width_ = 1;
     bool xcd_remap_ = true;
+    // per column thread: (chunk << 1) | reversed, device copy
+    int* kmap_ = nullptr;
+    int kmap_chunks_ = 0;
     std::vector<int> rad_row_, rad_col_;
 };
 

@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(512, (kColsSplit<T, LOGN, SYM> ? 4 : 1))
 k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int pitch,
           int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
           const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
-          double* __restrict__ part, int remap, RbBatch bt) {
+          double* __restrict__ part, int remap, RbBatch bt, const int* __restrict__ kmap) {
     using Sh = rb::Shape<LOGN>;
     {
         const int li = blockIdx.y, gi = bt.first + li;
@@ -255,8 +255,12 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
     // this thread are contiguous at kx*kp + 16*th (Gl*Nl = 16); loaded
     // inside the last forward exchange so the HBM latency is hidden
     T kv[16];
+    // half-size table: mirror-paired threads share a chunk, the upper one
+    // reads it reversed (kmap); without a map, one chunk per thread
+    int km = th << 1;
     if constexpr (SYM == 0) {
-        const T* kr = ktab + (size_t)(live ? kx : 0) * kp + 16 * th;
+        if (kmap) km = kmap[th];
+        const T* kr = ktab + (size_t)(live ? kx : 0) * kp + 16 * (km >> 1);
         rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1, [&]() { load16(kr, kv); });
     } else {
         rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1);
@@ -286,8 +290,12 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
         return;
     } else {
         if constexpr (SYM == 0) {
+            const bool rev = (km & 1) != 0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) r[i] = scale(r[i], live ? kv[i] : T(0));
+            for (int i = 0; i < 16; ++i) {
+                const T k = rev ? kv[15 - i] : kv[i];
+                r[i] = scale(r[i], live ? k : T(0));
+            }
         } else {
 #pragma unroll
             for (int q = 0; q < Gl; ++q)
@@ -523,7 +531,7 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
     }
     const dim3 grid(gx, a.bt.count);
 #define RC_ARGS a.AT, a.pitchT, a.BR, a.pitch, a.ncols, a.rows_in, a.rows_out, a.W, a.tw, a.ktab, \
-        a.kp, a.sc, a.kout, a.part, a.remap, a.bt
+        a.kp, a.sc, a.kout, a.part, a.remap, a.bt, a.kmap
 #define RC_CASE(S)                                                     \
     case S: {                                                          \
         const void* fn = (const void*)k_rb_cols<T, LOGN, S>;           \
@@ -710,6 +718,36 @@ void launch_permute(const T* kin, int kp, int nh, T* kout, hipStream_t s, bool o
     ASPHER_LAUNCH_CHECK();
 }
 
+template <typename T, int LOGN>
+__global__ void __launch_bounds__(256)
+k_rb_permute_table_half(const T* __restrict__ kin, int kp, int nh, const int* __restrict__ kmap,
+                        int kph, T* __restrict__ kout) {
+    using Sh = rb::Shape<LOGN>;
+    constexpr int n = Sh::n, Tn = Sh::threads;
+    constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
+    const std::int64_t total = (std::int64_t)nh * Tn * 16;
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i / (Tn * 16)), e = (int)(i - (std::int64_t)kx * (Tn * 16));
+        const int th = e >> 4, j = e & 15;
+        const int km = kmap[th];
+        if (km & 1) continue;   // this chunk is written by its forward reader
+        const int q = j / Nl, v = j - q * Nl;
+        const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
+        const int kyp = (ky <= (n >> 1)) ? ky : n - ky;
+        kout[(std::size_t)kx * kph + 16 * (km >> 1) + j] = kin[(std::size_t)kx * kp + kyp];
+    }
+}
+
+template <typename T, int LOGN>
+void launch_permute_half(const T* kin, int kp, int nh, const int* kmap, int kph, T* kout,
+                         hipStream_t s) {
+    const std::int64_t total = (std::int64_t)nh * (1 << LOGN);
+    const int grid = (int)std::min<std::int64_t>((total + 255) / 256, 65536);
+    k_rb_permute_table_half<T, LOGN><<<grid, 256, 0, s>>>(kin, kp, nh, kmap, kph, kout);
+    ASPHER_LAUNCH_CHECK();
+}
+
 // ── rough-surface generator: spectrum synthesis + inverse column pass ─────
 __device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned long long idx) {
     // SplitMix64 output number idx of a generator seeded with `seed`
@@ -882,6 +920,27 @@ void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream
         default: throw std::invalid_argument("column transform length out of range");
     }
 }
+template <typename T>
+void rb_permute_table_half(int logN, const T* kin, int kp, int nh, const int* kmap, int kph,
+                           T* kout, hipStream_t s) {
+    if (!kmap) throw std::invalid_argument("permute_table_half: no mirror map");
+    switch (logN) {
+        case 5: launch_permute_half<T, 5>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 6: launch_permute_half<T, 6>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 7: launch_permute_half<T, 7>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 8: launch_permute_half<T, 8>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 9: launch_permute_half<T, 9>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 10: launch_permute_half<T, 10>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 11: launch_permute_half<T, 11>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 12: launch_permute_half<T, 12>(kin, kp, nh, kmap, kph, kout, s); break;
+        case 13: launch_permute_half<T, 13>(kin, kp, nh, kmap, kph, kout, s); break;
+        default: throw std::invalid_argument("column transform length out of range");
+    }
+}
+template void rb_permute_table_half<float>(int, const float*, int, int, const int*, int, float*,
+                                           hipStream_t);
+template void rb_permute_table_half<double>(int, const double*, int, int, const int*, int,
+                                            double*, hipStream_t);
 template void rb_permute_table<float>(int, const float*, int, int, float*, hipStream_t, bool);
 template void rb_permute_table<double>(int, const double*, int, int, double*, hipStream_t,
                                        bool);

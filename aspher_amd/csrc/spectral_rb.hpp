@@ -58,6 +58,9 @@ struct RbCols {
     T* kout = nullptr;
     double* part = nullptr;
     int remap = 1;
+    // table symbol: per column thread (chunk << 1) | reversed into the
+    // half-size table (SpectralPlan::permute_table_half)
+    const int* kmap = nullptr;
     RbBatch bt;
 };
 
@@ -124,5 +127,11 @@ template <typename T> void rb_cols2(int logN, const RbCols2<T>& a, hipStream_t s
 template <typename T>
 void rb_permute_table(int logN, const T* kin, int kp, int nh, T* kout, hipStream_t s,
                       bool odd = false);
+// the half-size layout: kout[kx*kph + 16*chunk + i] = K(kx, ky) of register i
+// of the thread that reads chunk forward (kmap: per thread (chunk << 1) |
+// reversed)
+template <typename T>
+void rb_permute_table_half(int logN, const T* kin, int kp, int nh, const int* kmap, int kph,
+                           T* kout, hipStream_t s);
 
 } // namespace aspher
