@@ -450,6 +450,13 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
             throw std::invalid_argument("spectral tiling exceeds the workgroup limits");
         row_batch_ = B;
         col_width_ = W;
+        // the forward row pass writes the transposed half spectrum in runs of
+        // B rows: it takes as many rows as 1024 threads and 144 KiB of LDS
+        // allow (one resident workgroup), measured faster than two smaller
+        // ones; the inverse pass keeps B (more resident workgroups)
+        int Bf = tune.row_batch > 0 ? B : std::max(1, 1024 / (Lr / 16));
+        while (tune.row_batch <= 0 && Bf > B && Bf * rowlds > 147456) --Bf;
+        row_batch_fwd_ = std::max(Bf, B);
         pitchT_ = (std::max(rows_in, 1) + 31) / 32 * 32 + 32;
         tw64_ = std::make_unique<Twiddles<double>>(n);
         a_rows_ = static_cast<std::size_t>(std::max(rows_out, 1));
@@ -698,7 +705,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         f.nrows = rows_in;
         f.AT = AT_;
         f.pitchT = pitchT_;
-        f.B = row_batch_;
+        f.B = row_batch_fwd_;
         f.tw = tw64_->dev;
         f.mode = in_mode == RowIn::masked ? 1 : 0;
         f.mask = mask;
@@ -765,7 +772,7 @@ void SpectralPlan<T>::build_spectrum(const double* table, int Ns, T* kout, doubl
         f.nrows = n;
         f.AT = AT_;
         f.pitchT = pitchT_;
-        f.B = row_batch_;
+        f.B = row_batch_fwd_;
         f.tw = tw64_->dev;
         f.mode = embed;
         f.table = table;
@@ -817,7 +824,7 @@ void SpectralPlan<T>::convolve2(const T* x, int x_pitch, std::int64_t x_stride, 
     f.nrows = rows_in;
     f.AT = AT_;
     f.pitchT = pitchT_;
-    f.B = row_batch_;
+    f.B = row_batch_fwd_;
     f.tw = tw64_->dev;
     f.mode = in_mode == RowIn::masked ? 1 : 0;
     f.mask = mask;

@@ -169,6 +169,7 @@ private:
     int batch_ = 1;
     std::int64_t at_stride_ = 0, br_stride_ = 0;
     int row_batch_ = 1, col_width_ = 1;
+    int row_batch_fwd_ = 1;       // rows per workgroup of the forward row pass
     bool xcd_remap_ = true;
     // per column thread: (chunk << 1) | reversed, device copy
     int* kmap_ = nullptr;

@@ -45,7 +45,7 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
 // rows), so the upper half of the packed input is never loaded and the first
 // radix-16 stage is pruned.
 template <typename T, int LOGL, int IN, bool Z>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
               cx<T>* __restrict__ AT, int pitchT, int B, const cx<double>* __restrict__ tw,
               const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
