@@ -893,7 +893,7 @@ struct SetSpec {
 };
 
 __global__ void k_set_scal(double* scal, SetSpec s) {
-    if (threadIdx.x < s.n) scal[s.idx[threadIdx.x]] = s.val[threadIdx.x];
+    if ((int)threadIdx.x < s.n) scal[s.idx[threadIdx.x]] = s.val[threadIdx.x];
 }
 
 // ── Love kernel (src/boussinesq_kernel.cpp:44-95) ──────────────────────────
