@@ -151,7 +151,9 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     };
 
     long long mv = 0, pcount = 0;
-    EventTimer tm_mv, tm_pc;
+    // every eighth apply is timed: the applies of a level are alike, and each
+    // event record idles the device for a few microseconds
+    EventTimer tm_mv(8), tm_pc(8);
     auto sync_read = [&]() {
         ctx.read_scal();
         tm_mv.drain();

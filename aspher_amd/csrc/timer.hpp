@@ -10,17 +10,24 @@ namespace aspher {
 
 // Accumulates GPU time of (start, stop) event pairs without synchronising:
 // completed pairs are drained whenever the host has synchronised anyway.
+// With stride k only every k-th pair records events (each record leaves the
+// device idle for a few microseconds) and the total is extrapolated from
+// the timed ones.
 class EventTimer {
 public:
+    explicit EventTimer(int stride = 1) : stride_(stride < 1 ? 1 : stride) {}
     ~EventTimer() {
         for (auto& pr : pending_) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
         for (auto e : pool_) (void)hipEventDestroy(e);
     }
     void start(hipStream_t s) {
+        on_ = (calls_++ % stride_) == 0;
+        if (!on_) return;
         cur_ = get();
         (void)hipEventRecord(cur_, s);
     }
     void stop(hipStream_t s) {
+        if (!on_) return;
         hipEvent_t e = get();
         (void)hipEventRecord(e, s);
         pending_.emplace_back(cur_, e);
@@ -36,13 +43,20 @@ public:
                 continue;
             }
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) total_ += ms * 1e-3;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+                total_ += ms * 1e-3;
+                ++timed_;
+            }
             pool_.push_back(pr.first);
             pool_.push_back(pr.second);
         }
         pending_.resize(keep);
     }
-    double seconds() const { return total_; }
+    // all calls: the mean of the timed pairs taken so far times the number
+    // of (start, stop) calls (exact with stride 1 once every pair is drained)
+    double seconds() const {
+        return timed_ ? total_ * static_cast<double>(calls_) / static_cast<double>(timed_) : 0.0;
+    }
 
 private:
     hipEvent_t get() {
@@ -62,6 +76,9 @@ private:
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_;
     hipEvent_t cur_ = nullptr;
     double total_ = 0.0;
+    int stride_ = 1;
+    bool on_ = false;
+    long long calls_ = 0, timed_ = 0;
 };
 
 } // namespace aspher
