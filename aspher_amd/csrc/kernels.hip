@@ -812,8 +812,9 @@ struct FinSpec {
 
 // NV: the number of values (spec.nv) as a compile-time constant, so every
 // load of a batch is unconditional and the whole batch is in flight at once
-// (a runtime count left one memory round trip per value)
-template <typename T, int NV>
+// (a runtime count left one memory round trip per value). SUM: every value
+// is a sum, so the per-value choice of operation is not compiled in.
+template <typename T, int NV, bool SUM>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __restrict__ scal,
            int step, Ens ens) {
@@ -831,7 +832,7 @@ k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __re
     double acc[kPartVals];
 #pragma unroll
     for (int v = 0; v < kPartVals; ++v)
-        acc[v] = spec.ops[v] == RSUM ? 0.0
+        acc[v] = (SUM || spec.ops[v] == RSUM) ? 0.0
                  : (spec.ops[v] == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
     // rows b, b + 256, ... in that order; eight rows are loaded before any is
     // accumulated, so a batch costs one memory latency, not eight
@@ -852,7 +853,7 @@ k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __re
             for (int v = 0; v < kPartVals; ++v) {
                 if (v >= nv) break;
                 const double xv = x[u][v];
-                const int op = spec.ops[v];
+                const int op = SUM ? RSUM : spec.ops[v];
                 acc[v] = op == RSUM ? acc[v] + xv
                          : (op == RMIN ? ((xv < acc[v]) ? xv : acc[v])
                                        : ((xv > acc[v]) ? xv : acc[v]));
@@ -865,7 +866,7 @@ k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __re
 #pragma unroll
     for (int v = 0; v < kPartVals; ++v) {
         if (v >= nv) break;
-        const int op = spec.ops[v];
+        const int op = SUM ? RSUM : spec.ops[v];
         acc[v] = op == RSUM ? wave_reduce<RSUM>(acc[v])
                  : (op == RMIN ? wave_reduce<RMIN>(acc[v]) : wave_reduce<RMAX>(acc[v]));
         if (lane == 0) wred[wave * kPartVals + v] = acc[v];
@@ -873,7 +874,7 @@ k_finalize(const double* __restrict__ part, int nblk, FinSpec spec, double* __re
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int v = 0; v < nv; ++v) {
-            const int op = spec.ops[v];
+            const int op = SUM ? RSUM : spec.ops[v];
             double r = op == RSUM ? 0.0 : (op == RMIN ? __builtin_huge_val() : -__builtin_huge_val());
             for (int w = 0; w < nw; ++w) {
                 const double x = wred[w * kPartVals + v];
@@ -1377,11 +1378,16 @@ void finalize(const double* part, int nblk, int nv, const int* ops, const int* d
         f.ops[v] = ops[v];
         f.dst[v] = dst[v];
     }
+    bool all_sum = true;
+    for (int v = 0; v < nv; ++v) all_sum = all_sum && ops[v] == RSUM;
     const dim3 grid(std::max(ens.ninst, 1));
     const int st = static_cast<int>(step);
     switch (nv) {
 #define FIN_CASE(V) \
-        case V: k_finalize<T, V><<<grid, 256, 0, s>>>(part, nblk, f, scal, st, ens); break;
+        case V: \
+            if (all_sum) k_finalize<T, V, true><<<grid, 256, 0, s>>>(part, nblk, f, scal, st, ens); \
+            else k_finalize<T, V, false><<<grid, 256, 0, s>>>(part, nblk, f, scal, st, ens); \
+            break;
         FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4)
         FIN_CASE(5) FIN_CASE(6) FIN_CASE(7) FIN_CASE(8)
 #undef FIN_CASE
