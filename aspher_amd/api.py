@@ -127,6 +127,8 @@ class ContactResult:
 
     @property
     def timings(self) -> dict:
+        """Seconds per phase; ``matvec`` and ``precond`` are GPU time of every
+        eighth apply (event-timed), scaled to all applies."""
         i = self._i
         return dict(total=i.t_total, matvec=i.t_matvec, precond=i.t_precond, build=i.t_build,
                     coarse=i.t_coarse, verification=i.t_verification,

@@ -152,6 +152,8 @@ typedef struct {
     int64_t matvec_count, verification_matvec_count, precond_count;
     int identification_steps, returned_best, precond_dropped;
     int64_t mem_cg_state, mem_best_iterate, mem_contact_mask, mem_output_arrays, mem_peak;
+    /* seconds; t_matvec and t_precond are GPU time from events around every
+       eighth operator / preconditioner apply, scaled to all applies */
     double t_total, t_matvec, t_precond, t_build, t_coarse, t_verification, t_candidate,
         t_output;
     int has_displacement, has_gap;
