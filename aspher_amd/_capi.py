@@ -33,6 +33,7 @@ EXPORTS = (
     "aspher_tangential_create", "aspher_tangential_destroy", "aspher_tangential_matvec",
     "aspher_tangential_matvec_device", "aspher_tangential_precond_apply",
     "aspher_cerruti_points", "aspher_cerruti_tables",
+    "aspher_project_load_simplex", "aspher_test_fault",
 )
 
 
@@ -152,6 +153,8 @@ def _declare(L):
         "aspher_stencil_taps": (i, [i, i, ip, ip, ip, dp]),
         "aspher_precond_apply": (i, [i, s, i, i, dp, C.POINTER(C.c_uint8), dp]),
         "aspher_clear_cache": (i, []),
+        "aspher_project_load_simplex": (i, [C.c_int64, dp, d, i, dp, dp]),
+        "aspher_test_fault": (i, [i, C.c_int64]),
         "aspher_device_malloc": (i, [C.c_int64, C.POINTER(vp)]),
         "aspher_device_free": (i, [vp]),
         "aspher_memcpy_htod": (i, [vp, vp, C.c_int64]),

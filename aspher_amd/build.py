@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaspher_hip.so")
 SOURCES = ["spectral.hip", "spectral_rb.hip", "kernels.hip", "engine.hip", "solver.hip",
-           "active.hip", "adjoint.hip", "ensemble.hip", "tangential.hip", "capi.hip"]
+           "nested.hip", "active.hip", "adjoint.hip", "ensemble.hip", "tangential.hip", "capi.hip"]
 ARCH = os.environ.get("ASPHER_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code: only scalar and

@@ -723,6 +723,38 @@ int aspher_clear_cache(void) {
     return guard([&] { clear_operator_cache(); });
 }
 
+int aspher_project_load_simplex(int64_t n, const double* y, double P_total, int precision,
+                                double* out, double* theta) {
+    return guard([&] {
+        if (n <= 0) throw std::invalid_argument("project_load_simplex: n must be positive");
+        if (!y || !out) throw std::invalid_argument("null pointer");
+        if (!std::isfinite(P_total))
+            throw std::invalid_argument("project_load_simplex: P_total must be finite");
+        Context ctx;
+        hipStream_t st = ctx.stream;
+        DevBuf<double> yd((std::size_t)n);
+        ASPHER_HIP(hipMemcpyAsync(yd.p, y, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        double th = 0.0;
+        if (precision == 64) {
+            th = project_load_simplex_dev<double>(ctx, n, yd.p, P_total);
+        } else if (precision == 32) {
+            DevBuf<float> yf((std::size_t)n);
+            VecOps<float>::from_double(n, yd.p, 0.0, yf.p, st);
+            th = project_load_simplex_dev<float>(ctx, n, yf.p, P_total);
+            VecOps<float>::to_double(n, yf.p, yd.p, st);
+        } else {
+            throw std::invalid_argument("precision must be 64 or 32");
+        }
+        ASPHER_HIP(hipMemcpyAsync(out, yd.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        ASPHER_HIP(hipStreamSynchronize(st));
+        if (theta) *theta = th;
+    });
+}
+
+int aspher_test_fault(int mode, int64_t after) {
+    return guard([&] { fault_configure(mode, after); });
+}
+
 
 void aspher_adjoint_params_default(aspher_adjoint_params* p) {
     if (!p) return;

@@ -301,6 +301,7 @@ void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
             VecOps<T>::dense(Ns_, table_.p, x, y, s);
         }
     }
+    fault_after_apply<T>(y, s);
 }
 
 template <typename T>

@@ -271,6 +271,13 @@ struct Context {
     hipEvent_t scal_ready = nullptr;
 };
 
+// Load-simplex projection (src/contact_solver.cpp:31-60) of N device values
+// in place: y <- max(y - theta, 0) with sum = P_total, theta by Michelot's
+// threshold iteration (FP64 sums, at most 64 counting passes, one host read
+// each). Returns theta.
+template <typename Real>
+double project_load_simplex_dev(Context& ctx, std::int64_t N, Real* y, double P_total);
+
 // One certified PK solve (src/contact_solver.cpp:150-587) on device inputs:
 // g0 (N values of Real), optional p_init (consumed, may be modified).
 template <typename Real>

@@ -428,6 +428,7 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
             g.plan->convolve(x + off[g.first], g.Ns, y + off[g.first], g.Ns, sym, g.ktab,
                              g.kpitch, g.sym_scale, s, RowIn::plain, RowOut::plain, nullptr,
                              nullptr, 0, nullptr, nullptr, &sb);
+            fault_after_apply<Real>(y + off[g.first], s);
         }
     };
     auto precondition = [&]() {

@@ -12,7 +12,9 @@
 #include "reduce.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <stdexcept>
 
 namespace aspher {
 
@@ -1458,5 +1460,64 @@ void cast_d2f(std::int64_t N, const double* x, float* y, hipStream_t s) {
     k_cast_d2f<<<vec_blocks(N), kVecThreads, 0, s>>>(N, x, y);
     ASPHER_LAUNCH_CHECK();
 }
+
+// ── test hooks for the failure statuses ────────────────────────────────────
+
+namespace {
+
+std::atomic<int> g_fault_mode{FAULT_NONE};
+std::atomic<long long> g_fault_after{0};
+// applies counted on the device: a captured graph replays the counting
+// kernel, so replays count like eager launches
+unsigned long long* g_fault_count = nullptr;
+
+__global__ void k_fault_reset(unsigned long long* cnt) { *cnt = 0ULL; }
+
+template <typename T>
+__global__ void k_fault_nan(T* y, unsigned long long* cnt, long long after) {
+    const unsigned long long seen = atomicAdd(cnt, 1ULL);
+    if ((long long)seen >= after) y[0] = static_cast<T>(__builtin_nan(""));
+}
+
+template <typename T>
+__global__ void k_fault_drop(T* u, double drop) {
+    u[0] = static_cast<T>((double)u[0] - drop);
+}
+
+} // namespace
+
+void fault_configure(int mode, long long after) {
+    if (mode != FAULT_NONE && mode != FAULT_NONFINITE && mode != FAULT_FINAL_CHECK)
+        throw std::invalid_argument("fault mode must be 0 (off), 1 (nonfinite) or 2 (final_check)");
+    if (after < 0) throw std::invalid_argument("fault: after must be >= 0");
+    ASPHER_HIP(hipDeviceSynchronize());
+    if (mode == FAULT_NONFINITE) {
+        if (!g_fault_count) g_fault_count = static_cast<unsigned long long*>(dev_alloc(8));
+        k_fault_reset<<<1, 1>>>(g_fault_count);
+        ASPHER_LAUNCH_CHECK();
+        ASPHER_HIP(hipDeviceSynchronize());
+    }
+    g_fault_after.store(after);
+    g_fault_mode.store(mode);
+}
+
+int fault_mode() { return g_fault_mode.load(std::memory_order_relaxed); }
+
+template <typename T>
+void fault_after_apply(T* y, hipStream_t s) {
+    if (fault_mode() != FAULT_NONFINITE || !g_fault_count) return;
+    k_fault_nan<T><<<1, 1, 0, s>>>(y, g_fault_count, g_fault_after.load());
+    ASPHER_LAUNCH_CHECK();
+}
+template void fault_after_apply<double>(double*, hipStream_t);
+template void fault_after_apply<float>(float*, hipStream_t);
+
+template <typename T>
+void fault_before_final_check(T* u, double drop, hipStream_t s) {
+    k_fault_drop<T><<<1, 1, 0, s>>>(u, drop);
+    ASPHER_LAUNCH_CHECK();
+}
+template void fault_before_final_check<double>(double*, double, hipStream_t);
+template void fault_before_final_check<float>(float*, double, hipStream_t);
 
 } // namespace aspher

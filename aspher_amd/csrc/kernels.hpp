@@ -181,4 +181,21 @@ void copy16(std::int64_t bytes, const void* x, void* y, hipStream_t s);
 // float <-> double spectrum copy
 void cast_d2f(std::int64_t N, const double* x, float* y, hipStream_t s);
 
+// Test hooks for the failure statuses. The reference's tests reach them
+// through a wrapped operator functor (tests/test_certification.cpp:270-288);
+// the device paths have no functor seam, so the operator applies and the
+// final verification call these hooks, which do nothing unless a test has
+// switched them on through the C ABI (process-wide).
+//   FAULT_NONFINITE: every operator apply after the first `after` ones
+//     (counted on the device, so graph replays count too) writes NaN into
+//     element 0 of its output.
+//   FAULT_FINAL_CHECK: the final fresh evaluation of a converged single
+//     solve sees element 0 of its displacement lowered by 10 g_ref, as a
+//     non-deterministic operator would make it.
+enum FaultMode : int { FAULT_NONE = 0, FAULT_NONFINITE = 1, FAULT_FINAL_CHECK = 2 };
+void fault_configure(int mode, long long after);
+int fault_mode();
+template <typename T> void fault_after_apply(T* y, hipStream_t s);
+template <typename T> void fault_before_final_check(T* u, double drop, hipStream_t s);
+
 } // namespace aspher

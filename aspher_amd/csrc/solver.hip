@@ -1,6 +1,6 @@
-// Certified Polonsky–Keer projected CG (src/contact_solver.cpp:150-587) and
-// the nested-grid driver for the fft / periodic backends
-// (src/nested_solve.cpp:518-990), on device-resident state.
+// Certified Polonsky–Keer projected CG (src/contact_solver.cpp:150-587) on
+// device-resident state, and the load-simplex projection of its
+// identification step (src/contact_solver.cpp:31-60).
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "reduce.hpp"
@@ -52,6 +52,37 @@ Diag read_diag(const double* h) {
 }
 
 } // namespace
+
+template <typename Real>
+double project_load_simplex_dev(Context& ctx, std::int64_t N, Real* y, double P_total) {
+    if (N <= 0) return 0.0;
+    hipStream_t s = ctx.stream;
+    const int nblk = vec_blocks(N);
+    const double* hs = ctx.host_scal;
+    static const int d1[1] = {S_R0};
+    static const int d2[2] = {S_R0, S_R1};
+    VecOps<Real>::sum(N, y, ctx.part.p, nblk, s);
+    finalize<Real>(ctx.part.p, nblk, 1, kS1, d1, ctx.scal.p, STEP_NONE, s);
+    ctx.read_scal();
+    // Michelot: theta from the mean of the retained entries; each counting
+    // pass drops the entries at or below it (at most 64 passes)
+    double retained = (double)N;
+    double theta = (hs[S_R0] - P_total) / retained;
+    for (int pass = 0; pass < 64; ++pass) {
+        VecOps<Real>::count_above(N, y, theta, ctx.part.p, nblk, s);
+        finalize<Real>(ctx.part.p, nblk, 2, kS2, d2, ctx.scal.p, STEP_NONE, s);
+        ctx.read_scal();
+        const double above = hs[S_R1];
+        if (above == 0.0 || above == retained) break;
+        retained = above;
+        theta = (hs[S_R0] - P_total) / retained;
+    }
+    VecOps<Real>::project(N, y, theta, s);
+    return theta;
+}
+
+template double project_load_simplex_dev<double>(Context&, std::int64_t, double*, double);
+template double project_load_simplex_dev<float>(Context&, std::int64_t, float*, double);
 
 template <typename Real>
 void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
@@ -180,30 +211,6 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         set_scal(scal, 2, idx, val, s);
     };
 
-    // simplex projection (src/contact_solver.cpp:31-60), host-driven passes
-    auto project_load_simplex = [&](Real* y) {
-        static const int d1[1] = {S_R0};
-        static const int d2[2] = {S_R0, S_R1};
-        V::sum(N, y, part, nblk, s);
-        finalize<Real>(part, nblk, 1, kS1, d1, scal, STEP_NONE, s);
-        ctx.read_scal();
-        double ssum = hs[S_R0];
-        double nact = (double)N;
-        double theta = (ssum - P_total) / nact;
-        for (int pass = 0; pass < 64; ++pass) {
-            V::count_above(N, y, theta, part, nblk, s);
-            finalize<Real>(part, nblk, 2, kS2, d2, scal, STEP_NONE, s);
-            ctx.read_scal();
-            const double s2 = hs[S_R0], n2 = hs[S_R1];
-            if (n2 == 0.0) break;
-            if (n2 == nact) break;
-            nact = n2;
-            ssum = s2;
-            theta = (ssum - P_total) / nact;
-        }
-        V::project(N, y, theta, s);
-    };
-
     // feasible identification step (src/contact_solver.cpp:314-364)
     double s_id = 0.0;
     bool u_fresh = false;
@@ -217,7 +224,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         if (!(s_id > 0.0)) s_id = (pu > 0.0 && pp > 0.0) ? pp / pu : 1.0;
         for (int trial = 0; trial < 12; ++trial) {
             V::axpy_id(N, P, G, Z, s_id, s);
-            project_load_simplex(Z);
+            project_load_simplex_dev<Real>(ctx, N, Z, P_total);
             apply(Z, R);
             V::idcheck(N, Z, R, g0s, G, P, part, nblk, s);
             finalize<Real>(part, nblk, 2, kS2, d2, scal, STEP_NONE, s);
@@ -353,8 +360,13 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             pending = false;
             if (hs[S_OK] == 0.0) {
                 // the previous step had no usable curvature: run its
-                // identification step now, on the identical state
+                // identification step now, on the identical state. Its
+                // update left the pressure unchanged, so this iteration's
+                // apply re-applied the same pressure; the reference runs the
+                // identification step within the failed iteration and never
+                // makes that apply, so it is not counted.
                 discard_spec();
+                --mv;
                 if (!identification_step()) {
                     status = Status::nonpositive_curvature;
                     --it;
@@ -435,6 +447,8 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         }
     }
     if (!u_fresh) apply(P, U);
+    if (status == Status::converged && fault_mode() == FAULT_FINAL_CHECK)
+        fault_before_final_check<Real>(U, 10.0 * g_ref, s);
     evaluate();
     sync_read();
     d = read_diag(hs);
@@ -504,326 +518,5 @@ template void solve_impl<double>(Context&, Operator&, const double*, double,
                                  const SolveOptions&, Precond*, double*, Result&);
 template void solve_impl<float>(Context&, Operator&, const float*, float,
                                 const SolveOptions&, Precond*, float*, Result&);
-
-// ── nested driver (src/nested_solve.cpp:27-71,518-990; fft/periodic) ───────
-
-bool stencil_for_level(const std::string& engine, bool precond, double prev_occupancy,
-                       double occupancy_max) {
-    if (!precond) return false;
-    if (engine == "stencil") return true;
-    if (engine == "fft") return false;
-    return prev_occupancy < 0.0 || prev_occupancy < occupancy_max;
-}
-
-namespace {
-std::mutex g_cache_mu;
-std::map<std::tuple<int, int, double, double>, std::shared_ptr<Operator>> g_cache;
-} // namespace
-
-std::shared_ptr<Operator> cached_operator(Operator::Kind kind, int Ns, double L, double E,
-                                          hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    auto key = std::make_tuple((int)kind, Ns, L, E);
-    auto it = g_cache.find(key);
-    if (it != g_cache.end()) return it->second;
-    auto op = std::make_shared<Operator>(kind, Ns, L, E);
-    op->build(s);
-    g_cache[key] = op;
-    return op;
-}
-
-void clear_operator_cache() {
-    {
-        std::lock_guard<std::mutex> lk(g_cache_mu);
-        g_cache.clear();
-    }
-    // the freed operators' memory, and every other cached block, goes back
-    // to the runtime
-    (void)hipDeviceSynchronize();
-    dev_release_cached();
-}
-
-void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double p_bar,
-                  double tol, int max_iter, bool use_pr, const NestedParams& np, Result& res,
-                  bool reuse_operators) {
-    // input validation before any allocation or iteration (src/nested_solve.cpp:523-617)
-    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
-    if (Ns <= 0) throw std::invalid_argument("solve_contact_nested: Ns must be positive");
-    if (!(L > 0.0) || !std::isfinite(L))
-        throw std::invalid_argument("solve_contact_nested: L must be positive");
-    if (!(E_star > 0.0) || !std::isfinite(E_star))
-        throw std::invalid_argument("solve_contact_nested: E_star must be positive");
-    const std::int64_t N64 = (std::int64_t)Ns * Ns;
-    if (N64 > (std::int64_t)std::numeric_limits<int>::max())
-        throw std::invalid_argument("solve_contact_nested: Ns*Ns exceeds the int index range");
-    if (!(p_bar > 0.0) || !std::isfinite(p_bar))
-        throw std::invalid_argument("solve_contact_nested: p_nominal must be positive");
-    if (!(tol > 0.0) || !std::isfinite(tol))
-        throw std::invalid_argument("solve_contact_nested: tol must be positive");
-    if (max_iter < 0) throw std::invalid_argument("solve_contact_nested: max_iter must be >= 0");
-    if (np.coarsest <= 0)
-        throw std::invalid_argument("solve_contact_nested: coarsest must be positive");
-    if (np.coarsest > Ns) throw std::invalid_argument("solve_contact_nested: coarsest > Ns");
-    if (np.q < 2) throw std::invalid_argument("solve_contact_nested: q must be >= 2");
-    if (np.leaf_side <= 0 || !pow2(np.leaf_side))
-        throw std::invalid_argument(
-            "solve_contact_nested: leaf_side must be a positive power of two");
-    if (!(np.coarse_tol > 0.0))
-        throw std::invalid_argument("solve_contact_nested: coarse_tol must be positive");
-    if (np.active_set) {
-        if (np.active_halo < 0)
-            throw std::invalid_argument("solve_contact_nested: active_halo must be >= 0");
-        if (np.active_max_rounds < 1)
-            throw std::invalid_argument("solve_contact_nested: active_max_rounds must be >= 1");
-        if (!(np.active_delta >= 0.0))
-            throw std::invalid_argument("solve_contact_nested: active_delta must be >= 0");
-        if (!(np.active_occupancy_max > 0.0))
-            throw std::invalid_argument(
-                "solve_contact_nested: active_occupancy_max must be > 0");
-    }
-    if (!(np.precond_occupancy_max > 0.0))
-        throw std::invalid_argument("solve_contact_nested: precond_occupancy_max must be > 0");
-    std::string policy = np.precision;
-    if (policy.empty()) policy = np.single_precision ? "float" : "double";
-    if (policy != "double" && policy != "float" && policy != "float_then_double")
-        throw std::invalid_argument(
-            "precision must be 'double', 'float', 'float_then_double' or ''");
-    const bool any_float = policy != "double";
-    if (!(np.float_floor > 0.0))
-        throw std::invalid_argument("solve_contact_nested: float_floor must be positive");
-    // float_then_double with active_set: the reference rejects it for its h2
-    // backend; on the fft backend the float identification stage runs
-    // restricted and the double polish runs on the full grid.
-    if (policy == "float_then_double" && np.active_set && np.backend != "fft")
-        throw std::invalid_argument(
-            "solve_contact_nested: float_then_double does not yet route its polish through "
-            "the active-set path; use float_only or double_only with active_set=true");
-    std::vector<int> levels;
-    for (long long n = np.coarsest; n <= Ns; n *= 2) levels.push_back((int)n);
-    if (levels.empty() || levels.back() != Ns)
-        throw std::invalid_argument("solve_contact_nested: Ns must equal coarsest * 2^k");
-    if (np.backend != "h2" && np.backend != "fft" && np.backend != "periodic")
-        throw std::invalid_argument(
-            "solve_contact_nested: backend must be 'h2', 'fft', or 'periodic'");
-    if (np.active_set) {
-        // the restricted solve runs on the free-space operator (the
-        // reference's masked-matvec backend 'h2' is not part of this build)
-        if (np.backend != "fft")
-            throw std::invalid_argument(
-                "solve_contact_nested: active_set requires backend='fft' (or the reference's "
-                "'h2', not part of this build)");
-        if (levels.size() < 2)
-            throw std::invalid_argument(
-                "solve_contact_nested: active_set needs a coarse level (Ns > coarsest)");
-    }
-    if (np.backend == "h2")
-        throw std::invalid_argument(
-            "solve_contact_nested: backend 'h2' is not part of this build; use backend='fft' "
-            "(free space) or backend='periodic'");
-    if (np.precond_engine != "auto" && np.precond_engine != "stencil" &&
-        np.precond_engine != "fft")
-        throw std::invalid_argument("precond_engine must be 'auto', 'stencil' or 'fft'");
-
-    Context ctx;
-    hipStream_t s = ctx.stream;
-    const auto t_nested = Clock::now();
-    const int nblk = vec_blocks(N64);
-    {
-        static const int ops[3] = {RMIN, RMAX, RSUM};
-        static const int dst[3] = {S_R0, S_R1, S_R2};
-        VecOps<double>::stats(N64, gap_dev, ctx.part.p, nblk, s);
-        finalize<double>(ctx.part.p, nblk, 3, ops, dst, ctx.scal.p, STEP_NONE, s);
-        ctx.read_scal();
-    }
-    if (ctx.host_scal[S_R2] != 0.0)
-        throw std::invalid_argument("solve_contact_nested: gap contains non-finite values");
-    const double g_min_f = ctx.host_scal[S_R0], g_max_f = ctx.host_scal[S_R1];
-    const double datum = g_min_f + 0.5 * (g_max_f - g_min_f);
-    const double g_ref = std::max(g_max_f - g_min_f, p_bar * L / E_star);
-
-    const int ncoarse = (int)levels.size() - 1;
-    std::vector<DevBuf<double>> gap(ncoarse);
-    if (ncoarse > 0) {
-        gap[ncoarse - 1].alloc((std::size_t)levels[ncoarse - 1] * levels[ncoarse - 1]);
-        restrict2(Ns, gap_dev, gap[ncoarse - 1].p, s);
-        for (int i = ncoarse - 2; i >= 0; --i) {
-            gap[i].alloc((std::size_t)levels[i] * levels[i]);
-            restrict2(levels[i + 1], gap[i + 1].p, gap[i].p, s);
-        }
-    }
-
-    DevBuf<double> p_init;
-    DevBuf<double> coarse_gap;   // gap field of the level below (active_set only)
-    bool have_init = false;
-    double t_build = 0.0, t_coarse = 0.0, t_candidate_all = 0.0;
-    double prev_occupancy = -1.0;
-    std::vector<Stage> stages;
-    auto record_stage = [&](const std::string& name, const char* prec, double req, double eff,
-                            const Result& r) {
-        Stage st;
-        st.name = name;
-        st.precision = prec;
-        st.q = np.q;
-        st.requested_tol = req;
-        st.effective_tol = eff;
-        st.iterations = r.iterations;
-        st.matvec_count = r.matvec_count;
-        st.seconds = r.time_total;
-        st.status = r.status;
-        st.fw_error = r.fw_error;
-        st.penetration_error = r.penetration_error;
-        st.precond_dropped = r.precond_dropped;
-        stages.push_back(st);
-    };
-
-    const Operator::Kind kind =
-        np.backend == "fft" ? Operator::Kind::free_space : Operator::Kind::periodic;
-    for (std::size_t li = 0; li < levels.size(); ++li) {
-        const int n = levels[li];
-        const auto t_lvl = Clock::now();
-        std::shared_ptr<Operator> op;
-        if (reuse_operators) {
-            op = cached_operator(kind, n, L, E_star, s);
-        } else {
-            op = std::make_shared<Operator>(kind, n, L, E_star);
-            op->build(s);
-        }
-        const int r_lvl = std::min({np.precond_radius, std::max(1, n / 4), np.leaf_side});
-        const std::string engine =
-            (np.backend == "periodic" && np.precond_engine == "auto") ? std::string("fft")
-                                                                      : np.precond_engine;
-        const bool use_stencil =
-            stencil_for_level(engine, np.precond, prev_occupancy, np.precond_occupancy_max);
-        Precond pc(use_stencil ? Precond::Kind::stencil
-                               : (np.precond ? Precond::Kind::fourier : Precond::Kind::none),
-                   n, r_lvl);
-        t_build += since(t_lvl);
-
-        const bool finest = li + 1 == levels.size();
-        const double* glvl = finest ? gap_dev : gap[li].p;
-        const std::int64_t nn = (std::int64_t)n * n;
-        const bool polish_after = finest && policy == "float_then_double";
-        const bool level_float = any_float;
-        double lvl_tol = finest ? tol : np.coarse_tol;
-        if (polish_after) lvl_tol = np.float_floor;
-        else if (level_float) lvl_tol = std::max(lvl_tol, np.float_floor);
-        // a level whose successor will be restricted keeps its gap field:
-        // the successor's candidate set is built from it. Whether it will be
-        // restricted is predicted from this level's predecessor occupancy.
-        const bool gate_open_next =
-            prev_occupancy < 0.0 || prev_occupancy < np.active_occupancy_max;
-        const bool keep_gap = np.active_set && !finest && gate_open_next &&
-                              (np.active_all_levels || li + 2 == levels.size());
-        const bool light = finest ? np.light_result : !keep_gap;
-        const bool record_history = finest && np.record_error_history;
-        const bool use_active = np.active_set && have_init && coarse_gap.p != nullptr &&
-                                prev_occupancy < np.active_occupancy_max &&
-                                (finest || np.active_all_levels);
-
-        SolveOptions lopt;
-        lopt.tol = lvl_tol;
-        lopt.requested_tol = finest ? tol : np.coarse_tol;
-        lopt.max_iter = max_iter;
-        lopt.use_pr = use_pr;
-        lopt.light = light;
-        lopt.record_history = record_history;
-        lopt.keep_best = !light;
-        lopt.allow_tolerance_relaxation = !finest || polish_after || np.allow_tolerance_relaxation;
-        lopt.g_ref = g_ref;
-        lopt.datum = datum;
-        lopt.datum_mode = level_float ? SolveOptions::Datum::caller : SolveOptions::Datum::solver;
-        lopt.precond_cost_gate = np.precond_cost_gate;
-
-        Result lres;
-        if (use_active) {
-            if (level_float) {
-                op->build_single(s);
-                active_level<float>(ctx, *op, &pc, glvl, datum, p_bar, lvl_tol, max_iter, use_pr,
-                                    np, p_init, coarse_gap, record_history, light, g_ref, lres);
-            } else {
-                active_level<double>(ctx, *op, &pc, glvl, datum, p_bar, lvl_tol, max_iter,
-                                     use_pr, np, p_init, coarse_gap, record_history, light,
-                                     g_ref, lres);
-            }
-            if (!finest) gap[li].reset();
-            coarse_gap.reset();
-            lres.requested_tol = lopt.requested_tol;
-            t_candidate_all += lres.time_candidate;
-        } else if (level_float) {
-            op->build_single(s);
-            // the datum is removed inside the cast, before any rounding
-            DevBuf<float> g0f(nn);
-            VecOps<float>::from_double(nn, glvl, datum, g0f.p, s);
-            if (!finest) gap[li].reset();
-            DevBuf<float> p0f;
-            if (have_init) {
-                p0f.alloc(nn);
-                VecOps<float>::from_double(nn, p_init.p, 0.0, p0f.p, s);
-                p_init.reset();
-            }
-            solve_impl<float>(ctx, *op, g0f.p, static_cast<float>(p_bar), lopt, &pc,
-                              have_init ? p0f.p : nullptr, lres);
-        } else {
-            solve_impl<double>(ctx, *op, glvl, p_bar, lopt, &pc,
-                               have_init ? p_init.p : nullptr, lres);
-            if (!finest) gap[li].reset();
-        }
-        prev_occupancy = lres.contact_fraction;
-        record_stage(std::string(finest ? "finest:" : "coarse:") + std::to_string(n) +
-                         (use_active ? "(active)" : ""),
-                     level_float ? "float" : "double", lopt.requested_tol, lopt.tol, lres);
-
-        if (polish_after) {
-            // float identified the contact at the float floor; polish in
-            // double, warm-started, to the requested tolerance
-            if (!reuse_operators) op->release_single();
-            DevBuf<double> p_warm = std::move(lres.pressure);
-            SolveOptions popt;
-            popt.tol = tol;
-            popt.requested_tol = tol;
-            popt.max_iter = max_iter;
-            popt.use_pr = use_pr;
-            popt.light = np.light_result;
-            popt.record_history = record_history;
-            popt.keep_best = !np.light_result;
-            popt.allow_tolerance_relaxation = np.allow_tolerance_relaxation;
-            popt.g_ref = g_ref;
-            popt.datum = datum;
-            popt.datum_mode = SolveOptions::Datum::solver;
-            popt.precond_cost_gate = np.precond_cost_gate;
-            Result pres;
-            solve_impl<double>(ctx, *op, glvl, p_bar, popt, &pc, p_warm.p, pres);
-            record_stage("polish:" + std::to_string(n), "double", tol, tol, pres);
-            pres.iterations += lres.iterations;
-            pres.matvec_count += lres.matvec_count;
-            pres.precond_count += lres.precond_count;
-            pres.time_matvec += lres.time_matvec;
-            pres.time_precond += lres.time_precond;
-            pres.mem_peak = std::max(pres.mem_peak, lres.mem_peak);
-            if (record_history) {
-                std::vector<double> h = lres.error_history;
-                h.insert(h.end(), pres.error_history.begin(), pres.error_history.end());
-                pres.error_history = std::move(h);
-            }
-            lres = std::move(pres);
-        }
-
-        if (!finest) {
-            t_coarse += lres.time_total;
-            p_init.alloc((std::size_t)4 * nn);
-            prolong2(n, lres.pressure.p, p_init.p, s);
-            have_init = true;
-            if (keep_gap && lres.gap.p) coarse_gap = std::move(lres.gap);
-            else coarse_gap.reset();
-        }
-        if (finest) res = std::move(lres);
-    }
-    ASPHER_HIP(hipStreamSynchronize(s));
-    res.stage_stats = std::move(stages);
-    res.time_build = t_build;
-    res.time_coarse = t_coarse;
-    res.time_candidate = t_candidate_all;
-    res.time_total = since(t_nested);
-}
 
 } // namespace aspher

@@ -204,6 +204,31 @@ int aspher_precond_apply(int grid_size, const char* kind, int radius, int precis
 /* Drop the process-wide operator cache. */
 int aspher_clear_cache(void);
 
+/* project_load_simplex (src/contact_solver.cpp:31-60), the projection of the
+ * identification step, run on the device: out = max(y - theta, 0) with
+ * sum(out) = P_total, theta by Michelot's threshold iteration (FP64 sums, at
+ * most 64 passes). n values; precision 64 or 32 (host arrays are FP64 either
+ * way, the FP32 path rounds y first); theta may be NULL. The reference has no
+ * binding for it (its test T21 calls the C++ template,
+ * tests/test_certification.cpp:443-467); this entry point serves that test. */
+int aspher_project_load_simplex(int64_t n, const double* y, double P_total, int precision,
+                                double* out, double* theta);
+
+/* Test hook for the failure statuses `nonfinite` and `verification_failed`.
+ * The reference's tests inject faults through a wrapped MatVec functor
+ * (tests/test_certification.cpp:270-288); the device paths have no such
+ * seam, so this hook stands in for it. Process-wide, off by default:
+ *   mode 0: off.
+ *   mode 1: every operator apply after the first `after` ones, counted on the
+ *           device from this call on (graph replays included), writes NaN
+ *           into element 0 of its output; in an ensemble, into element 0 of
+ *           the first instance of each size group.
+ *   mode 2: the final fresh evaluation of every converged single-level
+ *           solve sees element 0 of its displacement lowered by 10 g_ref
+ *           (a non-deterministic operator), so it must report
+ *           verification_failed. */
+int aspher_test_fault(int mode, int64_t after);
+
 /* Same as aspher_solver_create, but the operator (Love table + kernel
  * spectrum) comes from the process-wide cache that solve_nested fills when
  * it reuses operators, so a later adjoint runs on the very spectrum of the

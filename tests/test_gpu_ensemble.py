@@ -133,3 +133,64 @@ def test_ensemble_uneven_sizes_c4_subset(hc):
         assert r.status == "converged", (pr["backend"], pr["grid_size"], r.status_reason)
         assert r.fw_error <= 1e-5 and r.penetration_error <= 1e-5
         assert abs(r.mean_pressure - pr["p_nominal"]) <= 1e-5 * pr["p_nominal"]
+
+
+def test_ensemble_c4_workload_certified_by_oracle(hc):
+    """The batched workload at its real sizes (64 instances drawn from
+    {256, 512, 1024, 2048}^2, Hurst in [0.6, 0.9], p_nominal log-uniform in
+    [0.001, 0.02], free/periodic; FP32 with FP64 reductions, tol 1e-5): every
+    returned pressure is certified by the oracle's FP64 operator at
+    1.01 max(tol, 2e-6), and has the load."""
+    import bench
+    probs = bench.ensemble_problems(hc)
+    tol = 1e-5
+    res, st = hc.solve_ensemble(probs, tol=tol)
+    print(st, flush=True)
+    assert {p["grid_size"] for p in probs} == {256, 512, 1024, 2048}
+    assert st["graph"] and st["host_fallbacks"] == 0
+    ops = {}
+    worst = {}
+    for pr, r in zip(probs, res):
+        key = (pr["backend"], pr["grid_size"])
+        Ns = pr["grid_size"]
+        assert r.status == "converged", (key, r.status, r.status_reason)
+        if key not in ops:
+            ops[key] = _oracle_op(*key)
+        d = O.diagnose(ops[key].matvec, pr["gap"], r.pressure.ravel(), pr["p_nominal"] * Ns * Ns,
+                       np.ptp(pr["gap"]))
+        w = worst.setdefault(key, [0.0, 0.0])
+        w[0] = max(w[0], d["fw_error"])
+        w[1] = max(w[1], d["penetration"])
+        gate = 1.01 * max(tol, 2e-6)
+        assert d["fw_error"] <= gate and d["penetration"] <= gate, (key, d)
+        assert d["load_error"] <= 1e-6 and d["min_pressure"] >= 0.0
+    for k in sorted(worst):
+        print("oracle certificate %s %d: fw %.3e pen %.3e" % (k + tuple(worst[k])), flush=True)
+
+
+def test_ensemble_fp64_1024_group_matches_single_solves(hc):
+    """FP64 ensemble of 1024^2 instances of the batched workload (both kinds)
+    against the single solves of the same problems and the oracle."""
+    import bench
+    probs = [p for p in bench.ensemble_problems(hc) if p["grid_size"] == 1024]
+    pick = [p for p in probs if p["backend"] == "fft"][:2] + \
+        [p for p in probs if p["backend"] == "periodic"][:1]
+    assert len(pick) == 3
+    tol = 1e-10
+    res, st = hc.solve_ensemble(pick, tol=tol, precision="double")
+    assert st["graph"]
+    for pr, r in zip(pick, res):
+        b, Ns = pr["backend"], pr["grid_size"]
+        assert r.status == "converged", (b, r.status, r.status_reason)
+        ref = hc.ContactSolver(Ns, backend=b).solve(
+            pr["gap"], pr["p_nominal"], tol=tol, max_iter=20000,
+            precond="fourier" if b == "fft" else "fourier-fft")
+        assert ref.converged
+        p, q = r.pressure.ravel(), ref.pressure.ravel()
+        rel = np.linalg.norm(p - q) / np.linalg.norm(q)
+        print(b, Ns, r.iterations, ref.iterations, "rel %.3e" % rel, flush=True)
+        assert rel < 1e-6
+        assert abs(r.contact_area - ref.contact_area) <= 1e-4
+        d = O.diagnose(_oracle_op(b, Ns).matvec, pr["gap"], p, pr["p_nominal"] * Ns * Ns,
+                       np.ptp(pr["gap"]))
+        assert d["fw_error"] <= 1.01 * tol and d["penetration"] <= 1.01 * tol, d

@@ -180,6 +180,29 @@ def test_preconditioners_offset_invariant_and_symmetric():
         assert abs(z0[contact].sum()) < 1e-12 * np.abs(z0).sum()
 
 
+@pytest.mark.parametrize("Ns", [16, 64, 96])
+def test_fourier_preconditioner_cos_mode_eigenvalues(Ns):
+    """On a full mask the preconditioner is F^-1 (|k|/Ns^2) F with unnormalised
+    transforms, so a cos mode m is an eigenvector with eigenvalue |kof(m)| =
+    hypot of the signed wavenumbers (src/fourier_precond.cpp:12-28),
+    Nyquist rows and columns included (kof(Ns/2) = -Ns/2, magnitude Ns/2).
+    A |k|^2 symbol, a wrong Nyquist kof or a dropped 1/Ns^2 all fail here;
+    the symbol is float, so the gate is its rounding (6e-8)."""
+    pc = O.FourierPreconditioner(Ns)
+    full = np.ones(Ns * Ns, dtype=bool)
+    iy, ix = np.meshgrid(np.arange(Ns), np.arange(Ns), indexing="ij")
+    h = Ns // 2
+    for mx, my in ((1, 0), (0, 1), (3, 5), (h, 0), (0, h), (h, h), (h - 1, h), (2, Ns - 3)):
+        g = np.cos(2 * math.pi * (mx * ix + my * iy) / Ns).ravel()
+        kx = mx if mx <= h else mx - Ns
+        ky = my if my < h else my - Ns
+        lam = math.hypot(kx, ky)
+        z = pc.apply(g, full)
+        assert np.linalg.norm(z - lam * g) <= 1e-7 * lam * np.linalg.norm(g), (mx, my)
+    # the DC mode is the null space (mean removal on both sides)
+    assert np.abs(pc.apply(np.full(Ns * Ns, 0.7), full)).max() == 0.0
+
+
 def test_stencil_for_level_table():
     """tests/test_precond.cpp:68-101."""
     f = O.stencil_for_level
