@@ -108,15 +108,24 @@ def test_fourier_preconditioner_cos_modes_4096(hc):
     preconditioner with eigenvalue |kof(m)| (symbol |k|/Ns^2 times the
     unnormalised inverse's Ns^2), including the Nyquist rows and columns,
     where kof(Ns/2) = -Ns/2 (src/fourier_precond.cpp:12-28). The symbol is
-    stored in float, so the FP64 gate is the float rounding of |k| (6e-8)."""
+    stored in float, so the FP64 gate is the float rounding of |k| (6e-8),
+    relative to the eigenvalue.
+
+    In FP32 the forward transform's rounding spreads about eps32 * ||g|| over
+    every mode, and the symbol multiplies each by up to max|kof| = Ns/sqrt(2),
+    so the FP32 error is bounded relative to the operator norm, not to a
+    small eigenvalue: ||z - lam g|| / (max|kof| ||g||) < 2e-6."""
     Ns = 4096
     mask = np.ones(Ns * Ns, dtype=bool)
     iy, ix = np.meshgrid(np.arange(Ns), np.arange(Ns), indexing="ij")
+    norm_op = math.hypot(Ns // 2, Ns // 2)
     for mx, my in ((1, 0), (3, 7), (Ns // 2, 0), (0, Ns // 2), (Ns // 2, Ns // 2),
                    (Ns // 2 - 1, 5)):
         g = np.cos(2 * math.pi * (mx * ix + my * iy) / Ns).ravel()
         lam = math.hypot(mx, my)
-        for prec, tol in ((64, 1e-7), (32, 2e-6)):
-            z = _precond("fourier-fft", Ns, g, mask, prec)
-            e = _rel(z, lam * g)
-            assert e < tol, (mx, my, prec, e)
+        z = _precond("fourier-fft", Ns, g, mask, 64)
+        e64 = _rel(z, lam * g)
+        assert e64 < 1e-7, (mx, my, 64, e64)
+        z32 = _precond("fourier-fft", Ns, g, mask, 32)
+        e32 = float(np.linalg.norm(z32 - lam * g) / (norm_op * np.linalg.norm(g)))
+        assert e32 < 2e-6, (mx, my, 32, e32)
