@@ -17,7 +17,8 @@ OK, ERR_INVALID, ERR_LOGIC, ERR_RUNTIME, ERR_DEVICE = 0, 1, 2, 3, 4
 EXPORTS = (
     "aspher_last_error", "aspher_version", "aspher_device_count",
     "aspher_solver_create", "aspher_solver_destroy", "aspher_solver_matvec",
-    "aspher_solver_matvec_device", "aspher_solver_time_matvec", "aspher_solver_time_stages",
+    "aspher_solver_matvec_device", "aspher_solver_matvec_device_noalloc",
+    "aspher_solver_time_matvec", "aspher_solver_time_stages",
     "aspher_solver_solve",
     "aspher_solver_solve_device", "aspher_solver_info", "aspher_nested_params_default",
     "aspher_solve_nested", "aspher_solve_nested_device", "aspher_result_get_info",
@@ -131,6 +132,7 @@ def _declare(L):
         "aspher_solver_destroy": (None, [vp]),
         "aspher_solver_matvec": (i, [vp, dp, dp]),
         "aspher_solver_matvec_device": (i, [vp, vp, vp, i]),
+        "aspher_solver_matvec_device_noalloc": (i, [vp, vp, vp, i]),
         "aspher_solver_time_matvec": (i, [vp, vp, vp, i, i, dp]),
         "aspher_solver_time_stages": (i, [vp, vp, vp, i, i, dp]),
         "aspher_solver_solve": (i, [vp, dp, d, d, i, i, s, dp, C.POINTER(vp)]),

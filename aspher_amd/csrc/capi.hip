@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -151,6 +152,10 @@ void create_solver(int grid_size, double domain_size, double E_star, const char*
         s->op = std::make_shared<Operator>(kind, grid_size, domain_size, E_star);
         s->op->build(s->ctx->stream);
     }
+    // FP64 plan and kernel spectrum in the plan's order, made here once so
+    // that later FP64 matvecs (the no-allocation one included) find them
+    s->op->prepare<double>(s->ctx->stream);
+    ASPHER_HIP(hipStreamSynchronize(s->ctx->stream));
     s->backend = b;
     s->Ns = grid_size;
     *out = s.release();
@@ -339,6 +344,66 @@ int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int pr
         } else {
             throw std::invalid_argument("precision must be 64 or 32");
         }
+        ASPHER_HIP(hipStreamSynchronize(st));
+    });
+}
+
+// ── matvec without allocation ─────────────────────────────────────────────
+// The intermediate spectra of aspher_solver_matvec_device_noalloc live in this
+// file-scope device array: each GPU the library runs on holds its own copy
+// (72 MiB) until the process ends. Host code never uses the name `scratch` as
+// an address (on the host it is only a stand-in for the symbol): the device
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current.
+constexpr std::size_t kNoAllocScratchBytes = std::size_t(72) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+
+// device address of `scratch` on the current device
+void* scratch_on_current_device() {
+    int dev = 0;
+    ASPHER_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        throw std::runtime_error("matvec without allocation: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") +
+                              hipGetErrorString(e));
+        g_scratch_dev[dev] = ptr;
+    }
+    return g_scratch_dev[dev];
+}
+} // namespace
+
+int aspher_solver_matvec_device_noalloc(aspher_solver* s, const void* x, void* y,
+                                        int precision) {
+    return guard([&] {
+        if (!s || !x || !y) throw std::invalid_argument("null pointer");
+        if (precision != 64 && precision != 32)
+            throw std::invalid_argument("precision must be 64 or 32");
+        const std::size_t need = precision == 64 ? s->op->workspace_external<double>()
+                                                 : s->op->workspace_external<float>();
+        if (need > kNoAllocScratchBytes)
+            throw std::invalid_argument(
+                "matvec without allocation: this grid needs " + std::to_string(need) +
+                " bytes of scratch, more than the static " +
+                std::to_string(kNoAllocScratchBytes) + " (72 MiB)");
+        void* ws = scratch_on_current_device();
+        ASPHER_HIP(hipDeviceSynchronize());
+        hipStream_t st = s->ctx->stream;
+        if (precision == 64)
+            s->op->apply_external(static_cast<const double*>(x), static_cast<double*>(y), ws,
+                                  kNoAllocScratchBytes, st);
+        else
+            s->op->apply_external(static_cast<const float*>(x), static_cast<float*>(y), ws,
+                                  kNoAllocScratchBytes, st);
         ASPHER_HIP(hipStreamSynchronize(st));
     });
 }

@@ -305,6 +305,85 @@ void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
 }
 
 template <typename T>
+bool Operator::prepared() const {
+    if (!built_ || kind_ == Kind::dense) return false;
+    const SpectralPlan<T>* pl;
+    const DevBuf<T>* perm;
+    if constexpr (std::is_same_v<T, double>) {
+        pl = plan64_.get();
+        perm = &khp64_;
+    } else {
+        pl = plan32_.get();
+        perm = &khp32_;
+    }
+    if (!pl || !pl->register_blocked()) return false;
+    return kind_ == Kind::periodic || perm->p != nullptr;
+}
+
+template <typename T>
+void Operator::prepare(hipStream_t s) {
+    if (!built_) throw std::logic_error("FFTOperator::prepare: build() not called");
+    if (kind_ == Kind::dense) return;
+    SpectralPlan<T>& pl = plan<T>(s);
+    if (kind_ == Kind::free_space) {
+        int kp = 0;
+        (void)table_for<T>(pl, s, kp);
+    }
+}
+template void Operator::prepare<double>(hipStream_t);
+template void Operator::prepare<float>(hipStream_t);
+
+template <typename T>
+std::size_t Operator::workspace_external() const {
+    const SpectralPlan<T>* pl;
+    if constexpr (std::is_same_v<T, double>) pl = plan64_.get();
+    else pl = plan32_.get();
+    return pl ? pl->external_ws_bytes() : 0;
+}
+
+template <typename T>
+void Operator::apply_external(const T* x, T* y, void* ws, std::size_t ws_bytes,
+                              hipStream_t s) {
+    if (kind_ == Kind::dense)
+        throw std::invalid_argument("matvec without allocation: spectral backends only "
+                                    "(fft, periodic)");
+    if (!prepared<T>())
+        throw std::logic_error(std::string("matvec without allocation: the operator is not "
+                                           "prepared in ") +
+                               (std::is_same_v<T, double> ? "FP64" : "FP32") +
+                               " (power-of-two grids 16..4096 free space, 32..8192 periodic; "
+                               "run one allocating matvec in this precision first)");
+    SpectralPlan<T>& pl = plan<T>(s);
+    const std::size_t need = pl.external_ws_bytes();
+    if (ws_bytes < need)
+        throw std::invalid_argument("matvec without allocation: needs " + std::to_string(need) +
+                                    " bytes of workspace, " + std::to_string(ws_bytes) +
+                                    " given");
+    if (kind_ == Kind::free_space) {
+        const T* kh;
+        if constexpr (std::is_same_v<T, double>) kh = khp64_.p;
+        else kh = khp32_.p;
+        pl.convolve(x, Ns_, y, Ns_, Symbol::table, kh, pl.table_pitch_half(), 0.0, s,
+                    RowIn::plain, RowOut::plain, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                    ws);
+    } else {
+        const double pi = std::acos(-1.0);
+        const double sc = L_ / (pi * E_ * static_cast<double>(Ns_) * Ns_);
+        pl.convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s, RowIn::plain,
+                    RowOut::plain, nullptr, nullptr, 0, nullptr, nullptr, nullptr, ws);
+    }
+    fault_after_apply<T>(y, s);
+}
+template void Operator::apply_external<double>(const double*, double*, void*, std::size_t,
+                                               hipStream_t);
+template void Operator::apply_external<float>(const float*, float*, void*, std::size_t,
+                                              hipStream_t);
+template bool Operator::prepared<double>() const;
+template bool Operator::prepared<float>() const;
+template std::size_t Operator::workspace_external<double>() const;
+template std::size_t Operator::workspace_external<float>() const;
+
+template <typename T>
 const T* Operator::table_for(const SpectralPlan<T>& pl, hipStream_t s, int& pitch) {
     if (kind_ != Kind::free_space || !built_)
         throw std::logic_error("table_for: free-space operator not built");

@@ -136,6 +136,21 @@ public:
     void apply_timed(const double* x, double* y, hipStream_t s, hipEvent_t* ev);
     void apply_timed(const float* x, float* y, hipStream_t s, hipEvent_t* ev);
 
+    // y = S x (spectral kinds) with the intermediate spectra in the caller's
+    // workspace `ws` (ws_bytes bytes) instead of the plan's own. Allocates
+    // nothing: the plan and, for free space, the kernel spectrum of this
+    // precision must already exist (`prepared`), else std::logic_error;
+    // a workspace smaller than workspace_external<T>() is invalid_argument.
+    template <typename T>
+    void apply_external(const T* x, T* y, void* ws, std::size_t ws_bytes, hipStream_t s);
+    template <typename T> bool prepared() const;
+    // set-up for apply_external in precision T (allocates: the plan and, for
+    // free space, the kernel spectrum in the plan's order)
+    template <typename T> void prepare(hipStream_t s);
+    // caller workspace bytes of apply_external (0 if the plan of this
+    // precision does not exist yet)
+    template <typename T> std::size_t workspace_external() const;
+
     Kind kind() const { return kind_; }
     int Ns() const { return Ns_; }
     double L() const { return L_; }

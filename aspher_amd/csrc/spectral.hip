@@ -509,6 +509,17 @@ SpectralPlan<T>::~SpectralPlan() {
 }
 
 template <typename T>
+std::size_t SpectralPlan<T>::ext_br_bytes() const {
+    return (sizeof(cx<T>) * (std::size_t)br_stride_ + 255) / 256 * 256;
+}
+
+template <typename T>
+std::size_t SpectralPlan<T>::external_ws_bytes() const {
+    if (!rb_) return 0;
+    return ext_br_bytes() + sizeof(cx<T>) * (std::size_t)at_stride_;
+}
+
+template <typename T>
 std::size_t SpectralPlan<T>::workspace_bytes() const {
     return sizeof(cx<T>) * a_rows_ * pitch * batch_ +
            (rb_ ? sizeof(cx<double>) * n + sizeof(cx<T>) * (std::size_t)nh * pitchT_ * batch_
@@ -681,11 +692,22 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
                                hipStream_t s, RowIn in_mode, RowOut out_mode,
                                const std::uint8_t* mask, const double* scal,
                                int gmean_idx, double* part, hipEvent_t* ev,
-                               const SpecBatch* sb) {
+                               const SpecBatch* sb, void* ext_ws) {
     if (sym == Symbol::build) throw std::invalid_argument("convolve: build symbol");
     if (sb && (!rb_ || sb->count > batch_ || sb->count < 1))
         throw std::invalid_argument("convolve: batch exceeds the plan");
+    if (ext_ws && (!rb_ || sb))
+        throw std::invalid_argument("convolve: caller workspace needs a single-instance "
+                                    "register-blocked plan");
     if (rb_) {
+        // intermediate spectra: the plan's own, or the caller's workspace
+        // (BR at its start, AT after it at a 256-byte boundary)
+        cx<T>* ws_br = A_;
+        cx<T>* ws_at = AT_;
+        if (ext_ws) {
+            ws_br = static_cast<cx<T>*>(ext_ws);
+            ws_at = reinterpret_cast<cx<T>*>(static_cast<char*>(ext_ws) + ext_br_bytes());
+        }
         RbBatch bt;
         if (sb) {
             bt.count = sb->count;
@@ -705,7 +727,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         f.x_pitch = x_pitch;
         f.ncols = cols_in;
         f.nrows = rows_in;
-        f.AT = AT_;
+        f.AT = ws_at;
         f.pitchT = pitchT_;
         f.B = row_batch_fwd_;
         f.tw = tw64_->dev;
@@ -717,9 +739,9 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         f.bt = bt;
         RbCols<T> c;
         c.bt = bt;
-        c.AT = AT_;
+        c.AT = ws_at;
         c.pitchT = pitchT_;
-        c.BR = A_;
+        c.BR = ws_br;
         c.pitch = pitch;
         c.ncols = nh;
         c.rows_in = rows_in;
@@ -733,7 +755,7 @@ void SpectralPlan<T>::convolve(const T* x, int x_pitch, T* y, int y_pitch, Symbo
         c.sc = sym_scale;
         c.remap = f.remap;
         RbRowsInv<T> r;
-        r.BR = A_;
+        r.BR = ws_br;
         r.pitch = pitch;
         r.nrows = rows_out;
         r.B = row_batch_;

@@ -92,7 +92,12 @@ public:
                   const std::uint8_t* mask = nullptr,
                   const double* scal = nullptr, int gmean_idx = 0,
                   double* part = nullptr, hipEvent_t* stage_events = nullptr,
-                  const SpecBatch* batch = nullptr);
+                  const SpecBatch* batch = nullptr, void* ext_ws = nullptr);
+    // Bytes of caller workspace that convolve(..., ext_ws) uses in place of
+    // the plan's own intermediate spectra (single instance, register-blocked
+    // plans; 0 for the generic path, which takes no caller workspace).
+    std::size_t external_ws_bytes() const;
+    bool register_blocked() const { return rb_; }
 
     // Forward-only pass used to build the free-space kernel spectrum from
     // the Love table (T = double): writes Re/n^2 into kout[(kx)(n/2+1)+ky]
@@ -146,6 +151,7 @@ public:
     int n, rows_in, cols_in, rows_out, cols_out, nh, pitch;
 
 private:
+    std::size_t ext_br_bytes() const;
     void rows_fwd(const T* x, int x_pitch, RowIn mode, const std::uint8_t* mask,
                   const double* scal, int gmean_idx, const double* table,
                   int Ns_embed, int nrows, int ncols, hipStream_t s);

@@ -59,6 +59,21 @@ int aspher_solver_matvec(aspher_solver* s, const double* p, double* u);
  * FP64 (precision 64) or FP32 (precision 32); x and y may alias. */
 int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int precision);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * u = S x as aspher_solver_matvec_device, on the same kernels, but the call
+ * allocates nothing. Its intermediate spectra live in one static 72 MiB
+ * device array per GPU, held until the process ends, which bounds the grid:
+ * free space up to Ns = 2048 in FP32 and 1024 in FP64, periodic up to
+ * Ns = 2048 in both (power-of-two grids; larger ones are rejected with
+ * ASPHER_ERR_INVALID). The handle must be prepared for the precision: FP64 is
+ * prepared by aspher_solver_create, FP32 by one aspher_solver_matvec_device
+ * call in FP32; otherwise ASPHER_ERR_LOGIC. Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other threads, other
+ * handles) race; calls in sequence are safe. x and y may alias. */
+int aspher_solver_matvec_device_noalloc(aspher_solver* s, const void* x, void* y,
+                                        int precision);
+
 /* Average device time (ms) of `reps` back-to-back device matvecs, measured
  * with HIP events on the library's own stream (benchmark support). */
 int aspher_solver_time_matvec(aspher_solver* s, const void* x, void* y, int precision,
