@@ -43,6 +43,9 @@ struct SpectralTuning {
     int row_batch = 0;   // rows per workgroup in the row kernels (0 = auto)
     int col_width = 0;   // columns per workgroup in the column kernel (0 = auto)
     bool xcd_remap = true;
+    // register-blocked plans: the column pass writes its result back into
+    // the transposed half spectrum and the inverse row pass reads it there
+    bool col_out_t = true;
 };
 
 // Equal-size instances of one convolve launch (ensemble groups, register-
@@ -144,7 +147,7 @@ public:
     void permute_table_half(const T* kunique, int kp, T* kout, hipStream_t s) const;
     void permute_table(const T* kunique, int kp, T* kout, hipStream_t s,
                        bool odd = false) const;
-    SpectralTuning tuning() const { return {row_batch_, col_width_, xcd_remap_}; }
+    SpectralTuning tuning() const { return {row_batch_, col_width_, xcd_remap_, tcols_}; }
     int col_blocks() const;       // blocks of cols (partial count, build)
     std::size_t workspace_bytes() const;
 
@@ -176,6 +179,8 @@ private:
     std::int64_t at_stride_ = 0, br_stride_ = 0;
     int row_batch_ = 1, col_width_ = 1;
     int row_batch_fwd_ = 1;       // rows per workgroup of the forward row pass
+    bool tcols_ = false;          // transposed column output (SpectralTuning::col_out_t)
+    int row_batch_inv_ = 1;       // rows per workgroup of the register-blocked inverse pass
     bool xcd_remap_ = true;
     // per column thread: (chunk << 1) | reversed, device copy
     int* kmap_ = nullptr;

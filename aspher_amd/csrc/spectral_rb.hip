@@ -203,16 +203,17 @@ constexpr bool kColsSplit = SYM == 0 && (sizeof(T) == 4 || LOGN >= 13);
 // workgroup spilled and was ~12% slower.
 template <typename T, int LOGN, int SYM>
 __global__ void __launch_bounds__(512, (kColsSplit<T, LOGN, SYM> ? 4 : 1))
-k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int pitch,
+k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
           int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
           const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
-          double* __restrict__ part, int remap, RbBatch bt, const int* __restrict__ kmap) {
+          double* __restrict__ part, int remap, RbBatch bt, const int* __restrict__ kmap,
+          int out_t) {
     using Sh = rb::Shape<LOGN>;
     {
         const int li = blockIdx.y, gi = bt.first + li;
         if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
         AT += li * bt.at_stride;
-        BR += li * bt.br_stride;
+        BR += li * (out_t ? bt.at_stride : bt.br_stride);
     }
     constexpr int n = Sh::n;
     constexpr int LL = rb::lds_len(n);
@@ -307,7 +308,18 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
                 }
         }
         rb::inverse<T, LOGN, ZP>(r, lds, th, tw, 1);
-        if (live) {
+        if (live && out_t) {
+            // transposed, in place: the column's rows [0, rows_out) back into
+            // its own AT column (every thread's loads of it were consumed
+            // before the first exchange barrier); consecutive lanes hold
+            // consecutive rows, so each store is one contiguous run
+            cx<T>* oc = BR + (size_t)kx * pitchT;
+#pragma unroll
+            for (int v = 0; v < (ZP ? 8 : 16); ++v) {
+                const int p = Sh::pos(0, th, v);
+                if (p < rows_out) oc[p] = r[v];
+            }
+        } else if (live) {
 #pragma unroll
             for (int v = 0; v < (ZP ? 8 : 16); ++v) {
                 const int p = Sh::pos(0, th, v);
@@ -318,20 +330,20 @@ k_rb_cols(const cx<T>* __restrict__ AT, int pitchT, cx<T>* __restrict__ BR, int 
     }
 }
 
-// ── rows: half spectrum (row-major) -> real ────────────────────────────────
+// ── rows: half spectrum (row-major or transposed) -> real ──────────────────
 // H: only the reals y[0 .. L) of n = 2L are wanted (free-space rows), so the
 // last radix-16 stage computes half of its outputs.
 template <typename T, int LOGL, int OUT, bool H>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
               const cx<double>* __restrict__ tw, T* __restrict__ y, int y_pitch, int ncols,
               const std::uint8_t* __restrict__ mask, double* __restrict__ part, int remap,
-              RbBatch bt) {
+              RbBatch bt, int in_t) {
     using Sh = rb::Shape<LOGL>;
     {
         const int li = blockIdx.y, gi = bt.first + li;
         if (bt.gate >= 0 && bt.gscal[(std::size_t)gi * bt.sstride + bt.gate] == 0.0) return;
-        BR += li * bt.br_stride;
+        BR += li * (in_t ? bt.at_stride : bt.br_stride);
         y += li * bt.y_stride;
         if (mask) mask += li * (bt.m_stride >= 0 ? bt.m_stride : bt.y_stride);
         if (part) part += (std::size_t)gi * bt.pstride;
@@ -353,18 +365,36 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
     constexpr int J = 17;
     const int lim = B * (L + 1);
     cx<T> xv[J];
+    if (in_t) {
+        // transposed input BR[k*pitch + row] (pitch = the column pitch): the
+        // workgroup's B rows fastest, so each k is one run of B values
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int e = threadIdx.x + j * (int)blockDim.x;
-        const int bb = e / (L + 1), k = e - bb * (L + 1);
-        const int rr = blk * B + bb;
-        xv[j] = (e < lim && rr < nrows) ? BR[(size_t)rr * pitch + k] : cx<T>{T(0), T(0)};
-    }
+        for (int j = 0; j < J; ++j) {
+            const int e = threadIdx.x + j * (int)blockDim.x;
+            const int k = e / B, bb = e - k * B;
+            const int rr = blk * B + bb;
+            xv[j] = (e < lim && rr < nrows) ? BR[(size_t)k * pitch + rr] : cx<T>{T(0), T(0)};
+        }
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int e = threadIdx.x + j * (int)blockDim.x;
-        const int bb = e / (L + 1), k = e - bb * (L + 1);
-        if (e < lim) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+        for (int j = 0; j < J; ++j) {
+            const int e = threadIdx.x + j * (int)blockDim.x;
+            const int k = e / B, bb = e - k * B;
+            if (e < lim) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int e = threadIdx.x + j * (int)blockDim.x;
+            const int bb = e / (L + 1), k = e - bb * (L + 1);
+            const int rr = blk * B + bb;
+            xv[j] = (e < lim && rr < nrows) ? BR[(size_t)rr * pitch + k] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int e = threadIdx.x + j * (int)blockDim.x;
+            const int bb = e / (L + 1), k = e - bb * (L + 1);
+            if (e < lim) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+        }
     }
     __syncthreads();
     // Z_k = (X_k + conj X_{L-k}) + i (X_k - conj X_{L-k}) exp(+2 pi i k/n),
@@ -531,7 +561,7 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
     }
     const dim3 grid(gx, a.bt.count);
 #define RC_ARGS a.AT, a.pitchT, a.BR, a.pitch, a.ncols, a.rows_in, a.rows_out, a.W, a.tw, a.ktab, \
-        a.kp, a.sc, a.kout, a.part, a.remap, a.bt, a.kmap
+        a.kp, a.sc, a.kout, a.part, a.remap, a.bt, a.kmap, a.out_t
 #define RC_CASE(S)                                                     \
     case S: {                                                          \
         const void* fn = (const void*)k_rb_cols<T, LOGN, S>;           \
@@ -541,6 +571,8 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
     }
     if (a.sym == 0 && (a.rows_in > n / 2 || a.rows_out > n / 2))
         throw std::invalid_argument("table symbol needs rows_in, rows_out <= n/2");
+    if (a.out_t && (a.BR != a.AT || a.sym == 3 || a.rows_out > a.pitchT))
+        throw std::invalid_argument("transposed column output is in place (BR == AT)");
     switch (a.sym) {
         RC_CASE(0)
         RC_CASE(1)
@@ -562,7 +594,7 @@ void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
     const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
     const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
 #define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap, \
-        a.bt
+        a.bt, a.in_t
     if (a.mode == 0 && a.ncols <= L) {
         const void* fn = (const void*)k_rb_rows_inv<T, LOGL, 0, true>;
         allow_lds_rb(fn, lds);

@@ -61,6 +61,9 @@ struct RbCols {
     // table symbol: per column thread (chunk << 1) | reversed into the
     // half-size table (SpectralPlan::permute_table_half)
     const int* kmap = nullptr;
+    // 1: the result goes back into the AT column it came from (BR == AT),
+    // transposed like the input, instead of row-major into BR
+    int out_t = 0;
     RbBatch bt;
 };
 
@@ -76,6 +79,9 @@ struct RbRowsInv {
     const std::uint8_t* mask = nullptr;
     double* part = nullptr;
     int remap = 1;
+    // 1: BR is the transposed half spectrum (value of row iy, frequency k
+    // at BR[k*pitch + iy], pitch the column pitch; instance stride at_stride)
+    int in_t = 0;
     RbBatch bt;
 };
 

@@ -64,8 +64,8 @@ int aspher_solver_matvec_device(aspher_solver* s, const void* x, void* y, int pr
  * u = S x as aspher_solver_matvec_device, on the same kernels, but the call
  * allocates nothing. Its intermediate spectra live in one static 72 MiB
  * device array per GPU, held until the process ends, which bounds the grid:
- * free space up to Ns = 2048 in FP32 and 1024 in FP64, periodic up to
- * Ns = 2048 in both (power-of-two grids; larger ones are rejected with
+ * free space up to Ns = 2048, periodic up to Ns = 4096 in FP32 and 2048 in
+ * FP64 (power-of-two grids; larger ones are rejected with
  * ASPHER_ERR_INVALID). The handle must be prepared for the precision: FP64 is
  * prepared by aspher_solver_create, FP32 by one aspher_solver_matvec_device
  * call in FP32; otherwise ASPHER_ERR_LOGIC. Every call on a GPU shares that

@@ -47,10 +47,10 @@ def _pair(hc, backend, Ns, prec, seed):
 
 
 @pytest.mark.parametrize("backend,sizes,prec", [
-    ("fft", (1024, 256), 64),
+    ("fft", (2048, 256), 64),
     ("fft", (2048, 512), 32),
     ("periodic", (2048, 64), 64),
-    ("periodic", (2048, 1024), 32),
+    ("periodic", (4096, 1024), 32),
 ])
 def test_noalloc_matches_allocating_matvec(hc, backend, sizes, prec):
     for k, Ns in enumerate(sizes):
@@ -82,9 +82,9 @@ def test_noalloc_rejections(hc):
     rc = L.aspher_solver_matvec_device_noalloc(s._h, C.c_void_p(xd.ptr), C.c_void_p(xd.ptr), 32)
     assert rc == _capi.ERR_LOGIC
     assert b"not prepared" in L.aspher_last_error()
-    # beyond the static scratch: free space FP64 at Ns = 2048 needs 130 MiB
-    s2 = hc.ContactSolver(2048, backend="fft")
-    xd2 = _dev(np.zeros(2048 * 2048), np.float64)
+    # beyond the static scratch: periodic FP64 at Ns = 4096 needs 129 MiB
+    s2 = hc.ContactSolver(4096, backend="periodic")
+    xd2 = _dev(np.zeros(4096 * 4096), np.float64)
     rc = L.aspher_solver_matvec_device_noalloc(s2._h, C.c_void_p(xd2.ptr), C.c_void_p(xd2.ptr),
                                                64)
     assert rc == _capi.ERR_INVALID
