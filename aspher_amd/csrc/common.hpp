@@ -107,6 +107,9 @@ struct Ens {
     const int* ns = nullptr;          // per-instance grid side (tile work items)
     int ninst = 1;                    // instances (finalize grid)
     const int* rows = nullptr;        // per-instance partial rows (finalize)
+    // every instance offset and work-item boundary is a multiple of 4, so
+    // the element loops may take 4 consecutive elements per access
+    bool vec4 = false;
 };
 
 } // namespace aspher

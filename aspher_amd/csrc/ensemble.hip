@@ -410,6 +410,10 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
     ev.ns = dns.p;
     ev.ninst = ni;
     ev.rows = dvrows.p;
+    // instance lengths Ns^2 (Ns >= 32) and chunks of kChunk elements:
+    // every boundary is a multiple of 4
+    ev.vec4 = kChunk % 4 == 0 &&
+              std::all_of(len.begin(), len.end(), [](std::int64_t l) { return l % 4 == 0; });
     Ens epc = ev;   // preconditioner partials
     epc.rows = dprow.p;
 

@@ -34,20 +34,24 @@ __device__ __forceinline__ void put(double v, double* red, double* part, int nv,
 // Ensemble prologue: the element range of this block (a work item of one
 // instance, or the grid-stride range of a single solve), the instance's
 // vector offset, and the gate on its activity flag.
-#define ENS_SETUP(N)                                                                   \
+#define ENS_SETUP(N) ENS_SETUP_V(N, 1)
+// VW consecutive elements per thread and step: the range boundaries (the
+// vector length, instance offsets, work items) must be multiples of VW,
+// which the launchers check before choosing VW > 1.
+#define ENS_SETUP_V(N, VW)                                                             \
     std::int64_t i0_, i1_, is_;                                                        \
     int inst_, row_;                                                                   \
     if (ens.work) {                                                                    \
         const WorkItem w_ = ens.work[blockIdx.x];                                      \
-        i0_ = w_.begin + threadIdx.x;                                                  \
+        i0_ = w_.begin + (std::int64_t)threadIdx.x * (VW);                             \
         i1_ = w_.end;                                                                  \
-        is_ = blockDim.x;                                                              \
+        is_ = (std::int64_t)blockDim.x * (VW);                                         \
         inst_ = w_.inst;                                                               \
         row_ = w_.row;                                                                 \
     } else {                                                                           \
-        i0_ = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x;                     \
+        i0_ = ((std::int64_t)blockIdx.x * blockDim.x + threadIdx.x) * (VW);            \
         i1_ = (N);                                                                     \
-        is_ = (std::int64_t)gridDim.x * blockDim.x;                                    \
+        is_ = (std::int64_t)gridDim.x * blockDim.x * (VW);                             \
         inst_ = 0;                                                                     \
         row_ = blockIdx.x;                                                             \
     }                                                                                  \
@@ -60,6 +64,56 @@ __device__ __forceinline__ void put(double v, double* red, double* part, int nv,
 #define ENS_SCAL(p) ((p) + (std::size_t)inst_ * ens.sstride)
 #define ENS_PART(p) ((p) + (std::size_t)inst_ * ens.pstride + (std::size_t)row_ * kPartVals)
 
+// VW consecutive values from an aligned address: one 16-byte access per 4
+// FP32 values or 2 FP64 values, one 4-byte access per 4 mask bytes
+template <int VW>
+__device__ __forceinline__ void ldv(const float* p, float* v) {
+    if constexpr (VW == 1) {
+        v[0] = p[0];
+    } else {
+        static_assert(VW == 4, "vector width 1 or 4");
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    }
+}
+template <int VW>
+__device__ __forceinline__ void ldv(const double* p, double* v) {
+    if constexpr (VW == 1) {
+        v[0] = p[0];
+    } else {
+        static_assert(VW == 4, "vector width 1 or 4");
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 b = *reinterpret_cast<const double2*>(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+}
+template <int VW>
+__device__ __forceinline__ void stv(float* p, const float* v) {
+    if constexpr (VW == 1) p[0] = v[0];
+    else *reinterpret_cast<float4*>(p) = float4{v[0], v[1], v[2], v[3]};
+}
+template <int VW>
+__device__ __forceinline__ void stv(double* p, const double* v) {
+    if constexpr (VW == 1) {
+        p[0] = v[0];
+    } else {
+        *reinterpret_cast<double2*>(p) = double2{v[0], v[1]};
+        *reinterpret_cast<double2*>(p + 2) = double2{v[2], v[3]};
+    }
+}
+// mask bytes as one word (byte k = element k)
+template <int VW>
+__device__ __forceinline__ unsigned ldm(const std::uint8_t* p) {
+    if constexpr (VW == 1) return p[0];
+    else return *reinterpret_cast<const unsigned*>(p);
+}
+template <int VW>
+__device__ __forceinline__ void stm(std::uint8_t* p, unsigned m) {
+    if constexpr (VW == 1) p[0] = (std::uint8_t)m;
+    else *reinterpret_cast<unsigned*>(p) = m;
+}
+__device__ __forceinline__ bool mbit(unsigned m, int k) { return ((m >> (8 * k)) & 0xffu) != 0; }
+
 template <int OP>
 __device__ __forceinline__ void putr(double v, double* red, double* prow, int idx) {
     v = block_reduce<OP>(v, red);
@@ -68,37 +122,45 @@ __device__ __forceinline__ void putr(double v, double* red, double* prow, int id
 
 // ── evaluate (src/contact_solver.cpp:260-303) ──────────────────────────────
 
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_eval1(std::int64_t N, const T* __restrict__ u, const T* __restrict__ g0s,
         const T* __restrict__ p, T* __restrict__ g, double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     u += o_; g0s += o_; p += o_; g += o_;
     double* pr = ENS_PART(part);
     double gsum = 0.0, nc = 0.0, vmin = __builtin_huge_val();
     ENS_LOOP(i) {
-        const T v = u[i] + g0s[i];
-        g[i] = v;
-        const double vd = (double)v;
-        if (vd < vmin) vmin = vd;
-        if (p[i] > T(0)) {
-            gsum += vd;
-            nc += 1.0;
+        T uv[VW], gv[VW], pv[VW];
+        ldv<VW>(u + i, uv);
+        ldv<VW>(g0s + i, gv);
+        ldv<VW>(p + i, pv);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            const T v = uv[k] + gv[k];
+            gv[k] = v;
+            const double vd = (double)v;
+            if (vd < vmin) vmin = vd;
+            if (pv[k] > T(0)) {
+                gsum += vd;
+                nc += 1.0;
+            }
         }
+        stv<VW>(g + i, gv);
     }
     putr<RSUM>(gsum, red, pr, 0);
     putr<RSUM>(nc, red, pr, 1);
     putr<RMIN>(vmin, red, pr, 2);
 }
 
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
         std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
         double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     p += o_; g += o_; mask += o_;
     scal = ENS_SCAL(scal);
     double* pr = ENS_PART(part);
@@ -106,19 +168,28 @@ k_eval2(std::int64_t N, const T* __restrict__ p, T* __restrict__ g,
     const T gmin = static_cast<T>(scal[S_GMIN]);
     double pv = 0.0, gcs = 0.0, e = 0.0, Gc = 0.0, psum = 0.0;
     ENS_LOOP(i) {
-        const T pi = p[i];
-        const double pd = (double)pi;
-        T gi = g[i];
-        pv += pd * (double)gi;
-        gi = gi - alpha;
-        g[i] = gi;
-        const double gd = (double)gi;
-        const bool c = pi > T(0);
-        if (c) gcs += gd;
-        e += pd * fabs(gd);
-        Gc += pd * (double)(T)(gi - gmin);
-        psum += pd;
-        mask[i] = c ? 1 : 0;
+        T pa[VW], ga[VW];
+        ldv<VW>(p + i, pa);
+        ldv<VW>(g + i, ga);
+        unsigned m = 0u;
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            const T pi = pa[k];
+            const double pd = (double)pi;
+            T gi = ga[k];
+            pv += pd * (double)gi;
+            gi = gi - alpha;
+            ga[k] = gi;
+            const double gd = (double)gi;
+            const bool c = pi > T(0);
+            if (c) gcs += gd;
+            e += pd * fabs(gd);
+            Gc += pd * (double)(T)(gi - gmin);
+            psum += pd;
+            m |= (c ? 1u : 0u) << (8 * k);
+        }
+        stv<VW>(g + i, ga);
+        stm<VW>(mask + i, m);
     }
     putr<RSUM>(pv, red, pr, 0);
     putr<RSUM>(gcs, red, pr, 1);
@@ -290,13 +361,13 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
 
 // ── CG direction (src/contact_solver.cpp:411-440) ──────────────────────────
 
-template <typename T, bool PC>
+template <typename T, bool PC, int VW>
 __global__ void __launch_bounds__(256)
 k_cgdot(std::int64_t N, const T* __restrict__ g, const T* __restrict__ gprev,
         const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
         const double* __restrict__ scal, double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     g += o_; gprev += o_; mask += o_;
     if (PC) z += o_;
     scal = ENS_SCAL(scal);
@@ -304,25 +375,34 @@ k_cgdot(std::int64_t N, const T* __restrict__ g, const T* __restrict__ gprev,
     const T zmean = PC ? static_cast<T>(scal[S_ZMEAN]) : T(0);
     double G = 0.0, Gpr = 0.0, gg = 0.0;
     ENS_LOOP(i) {
-        if (!mask[i]) continue;
-        const T gi = g[i];
-        const T zi = PC ? T(z[i] - zmean) : gi;
-        const double gd = (double)gi, zd = (double)zi;
-        G += zd * gd;
-        Gpr += zd * (double)(T)(gi - gprev[i]);
-        gg += gd * gd;
+        const unsigned m = ldm<VW>(mask + i);
+        if (m == 0u) continue;
+        T ga[VW], gpa[VW], za[VW];
+        ldv<VW>(g + i, ga);
+        ldv<VW>(gprev + i, gpa);
+        if (PC) ldv<VW>(z + i, za);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            if (!mbit(m, k)) continue;
+            const T gi = ga[k];
+            const T zi = PC ? T(za[k] - zmean) : gi;
+            const double gd = (double)gi, zd = (double)zi;
+            G += zd * gd;
+            Gpr += zd * (double)(T)(gi - gpa[k]);
+            gg += gd * gd;
+        }
     }
     putr<RSUM>(G, red, pr, 0);
     putr<RSUM>(Gpr, red, pr, 1);
     putr<RSUM>(gg, red, pr, 2);
 }
 
-template <typename T, bool PC>
+template <typename T, bool PC, int VW>
 __global__ void __launch_bounds__(256)
 k_tupdate(std::int64_t N, const T* __restrict__ g, T* __restrict__ gprev,
           const T* __restrict__ z, const std::uint8_t* __restrict__ mask,
           T* __restrict__ t, const double* __restrict__ scal, Ens ens) {
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     g += o_; mask += o_; t += o_;
     if (gprev) gprev += o_;
     if (PC) z += o_;
@@ -332,71 +412,109 @@ k_tupdate(std::int64_t N, const T* __restrict__ g, T* __restrict__ gprev,
     // gprev == nullptr: the caller keeps g_prev by rotating buffers (no copy)
     const bool copy = gprev != nullptr;
     ENS_LOOP(i) {
-        if (mask[i]) {
-            const T zi = PC ? T(z[i] - zmean) : g[i];
-            t[i] = zi + beta * t[i];
+        const unsigned m = ldm<VW>(mask + i);
+        T ta[VW];
+        if (m == 0u) {
+#pragma unroll
+            for (int k = 0; k < VW; ++k) ta[k] = T(0);
         } else {
-            t[i] = T(0);
+            T za[VW];
+            ldv<VW>(t + i, ta);
+            if (PC) ldv<VW>(z + i, za);
+            else ldv<VW>(g + i, za);
+#pragma unroll
+            for (int k = 0; k < VW; ++k) {
+                const T zi = PC ? T(za[k] - zmean) : za[k];
+                ta[k] = mbit(m, k) ? T(zi + beta * ta[k]) : T(0);
+            }
         }
-        if (copy) gprev[i] = g[i];
+        stv<VW>(t + i, ta);
+        if (copy) {
+            T ga[VW];
+            ldv<VW>(g + i, ga);
+            stv<VW>(gprev + i, ga);
+        }
     }
 }
 
 // ── exact line search (src/contact_solver.cpp:446-459) ─────────────────────
 
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_line1(std::int64_t N, const T* __restrict__ r, const T* __restrict__ g,
         const T* __restrict__ t, const std::uint8_t* __restrict__ mask,
         double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     r += o_; g += o_; t += o_; mask += o_;
     double* pr = ENS_PART(part);
     double rsum = 0.0, num = 0.0;
     ENS_LOOP(i) {
-        if (!mask[i]) continue;
-        rsum += (double)r[i];
-        num += (double)g[i] * (double)t[i];
+        const unsigned m = ldm<VW>(mask + i);
+        if (m == 0u) continue;
+        T ra[VW], ga[VW], ta[VW];
+        ldv<VW>(r + i, ra);
+        ldv<VW>(g + i, ga);
+        ldv<VW>(t + i, ta);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            if (!mbit(m, k)) continue;
+            rsum += (double)ra[k];
+            num += (double)ga[k] * (double)ta[k];
+        }
     }
     putr<RSUM>(rsum, red, pr, 0);
     putr<RSUM>(num, red, pr, 1);
 }
 
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_line2(std::int64_t N, const T* __restrict__ r, const T* __restrict__ t,
         const std::uint8_t* __restrict__ mask, const double* __restrict__ scal,
         double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     r += o_; t += o_; mask += o_;
     scal = ENS_SCAL(scal);
     double* pr = ENS_PART(part);
     const T rmean = static_cast<T>(scal[S_RMEAN]);
     double den = 0.0;
     ENS_LOOP(i) {
-        if (!mask[i]) continue;
-        den += (double)(T)(r[i] - rmean) * (double)t[i];
+        const unsigned m = ldm<VW>(mask + i);
+        if (m == 0u) continue;
+        T ra[VW], ta[VW];
+        ldv<VW>(r + i, ra);
+        ldv<VW>(t + i, ta);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            if (!mbit(m, k)) continue;
+            den += (double)(T)(ra[k] - rmean) * (double)ta[k];
+        }
     }
     putr<RSUM>(den, red, pr, 0);
 }
 
 // ── update, overlap step, load rescale (src/contact_solver.cpp:477-501) ────
 
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_update(std::int64_t N, const T* pin, T* pout, const T* __restrict__ t,
          const T* __restrict__ g, const double* __restrict__ scal,
          double* __restrict__ part, Ens ens) {
     __shared__ double red[8];
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     pin += o_; pout += o_; t += o_; g += o_;
     scal = ENS_SCAL(scal);
     double* pr = ENS_PART(part);
     if (scal[S_OK] == 0.0) {
         // no usable curvature: the pressure is unchanged
-        if (pout != pin) ENS_LOOP(i) pout[i] = pin[i];
+        if (pout != pin) {
+            ENS_LOOP(i) {
+                T pa[VW];
+                ldv<VW>(pin + i, pa);
+                stv<VW>(pout + i, pa);
+            }
+        }
         if (threadIdx.x == 0) {
             pr[0] = 0.0;
             pr[1] = 0.0;
@@ -407,45 +525,72 @@ k_update(std::int64_t N, const T* pin, T* pout, const T* __restrict__ t,
     const T tau_ov = static_cast<T>(scal[S_TAUOV]);
     double changed = 0.0, total = 0.0;
     ENS_LOOP(i) {
-        T pi = pin[i];
-        const bool was = pi > T(0);
-        if (was) {
-            const T q = pi - tau * t[i];
-            pi = (q < T(0)) ? T(0) : q;  // std::max(q, 0): NaN propagates
+        T pa[VW], ga[VW], ta[VW];
+        ldv<VW>(pin + i, pa);
+        ldv<VW>(g + i, ga);
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < VW; ++k) any = any || pa[k] > T(0);
+        // t is zero off the contact set: read only where some p > 0
+        if (any) ldv<VW>(t + i, ta);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            T pi = pa[k];
+            const bool was = pi > T(0);
+            if (was) {
+                const T q = pi - tau * ta[k];
+                pi = (q < T(0)) ? T(0) : q;  // std::max(q, 0): NaN propagates
+            }
+            const T gi = ga[k];
+            if (pi == T(0) && gi < T(0)) pi = pi - tau_ov * gi;
+            if ((pi > T(0)) != was) changed = 1.0;
+            pa[k] = pi;
+            total += (double)pi;
         }
-        const T gi = g[i];
-        if (pi == T(0) && gi < T(0)) pi = pi - tau_ov * gi;
-        if ((pi > T(0)) != was) changed = 1.0;
-        pout[i] = pi;
-        total += (double)pi;
+        stv<VW>(pout + i, pa);
     }
     putr<RMAX>(changed, red, pr, 0);
     putr<RSUM>(total, red, pr, 1);
 }
 
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_rescale(std::int64_t N, T* __restrict__ p, const double* __restrict__ scal, Ens ens) {
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     p += o_;
     scal = ENS_SCAL(scal);
     if (scal[S_OK] == 0.0) return;
     const bool fill = scal[S_PFILL] != 0.0;
     const T f = static_cast<T>(scal[S_F]);
     const T pbar = static_cast<T>(scal[S_PBAR]);
-    ENS_LOOP(i) p[i] = fill ? pbar : T(p[i] * f);
+    ENS_LOOP(i) {
+        T pa[VW];
+        if (fill) {
+#pragma unroll
+            for (int k = 0; k < VW; ++k) pa[k] = pbar;
+        } else {
+            ldv<VW>(p + i, pa);
+#pragma unroll
+            for (int k = 0; k < VW; ++k) pa[k] = T(pa[k] * f);
+        }
+        stv<VW>(p + i, pa);
+    }
 }
 
 // best iterate: y = x where the instance's scalar `flag` is nonzero
-template <typename T>
+template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_copy_if(std::int64_t N, const T* __restrict__ x, T* __restrict__ y,
           const double* __restrict__ scal, int flag, Ens ens) {
-    ENS_SETUP(N);
+    ENS_SETUP_V(N, VW);
     x += o_; y += o_;
     scal = ENS_SCAL(scal);
     if (scal[flag] == 0.0) return;
-    ENS_LOOP(i) y[i] = x[i];
+    ENS_LOOP(i) {
+        T a[VW];
+        ldv<VW>(x + i, a);
+        stv<VW>(y + i, a);
+    }
 }
 
 // ── identification step and simplex projection (src/contact_solver.cpp:31-60,314-364)
@@ -1161,6 +1306,14 @@ k_copy16(std::int64_t n16, const float4* __restrict__ x, float4* __restrict__ y)
 } // namespace
 
 #define L1(k, ...) k<<<nblk, kVecThreads, 0, s>>>(__VA_ARGS__); ASPHER_LAUNCH_CHECK()
+// 4 consecutive elements per access when every range boundary is a multiple
+// of 4 (single vector: N; ensemble: flagged by its builder) and every
+// pointer is 16-byte aligned (null pointers are not read)
+inline bool a16(const void* p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; }
+template <typename... P>
+bool vec4_ok(std::int64_t N, const Ens& ens, P... ptrs) {
+    return N % 4 == 0 && (!ens.work || ens.vec4) && (a16(ptrs) && ...);
+}
 #define L1S(k, ...) k<<<vec_blocks(N), kVecThreads, 0, s>>>(__VA_ARGS__); ASPHER_LAUNCH_CHECK()
 // ensemble-aware launches: one block per work item, else the single-solve grid
 #define LE(k, ...) k<<<ens.work ? ens.nwork : nblk, kVecThreads, 0, s>>>(__VA_ARGS__); \
@@ -1171,13 +1324,15 @@ k_copy16(std::int64_t n16, const float4* __restrict__ x, float4* __restrict__ y)
 template <typename T>
 void VecOps<T>::eval1(std::int64_t N, const T* u, const T* g0s, const T* p, T* g,
                       double* part, int nblk, hipStream_t s, const Ens& ens) {
-    LE(k_eval1<T>, N, u, g0s, p, g, part, ens);
+    if (vec4_ok(N, ens, u, g0s, p, g)) { LE((k_eval1<T, 4>), N, u, g0s, p, g, part, ens); }
+    else { LE((k_eval1<T, 1>), N, u, g0s, p, g, part, ens); }
 }
 template <typename T>
 void VecOps<T>::eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
                       const double* scal, double* part, int nblk, hipStream_t s,
                       const Ens& ens) {
-    LE(k_eval2<T>, N, p, g, mask, scal, part, ens);
+    if (vec4_ok(N, ens, p, g, mask)) { LE((k_eval2<T, 4>), N, p, g, mask, scal, part, ens); }
+    else { LE((k_eval2<T, 1>), N, p, g, mask, scal, part, ens); }
 }
 template <typename T>
 void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const double* scal,
@@ -1203,43 +1358,54 @@ template <typename T>
 void VecOps<T>::cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
                       const std::uint8_t* mask, const double* scal, bool has_pc,
                       double* part, int nblk, hipStream_t s, const Ens& ens) {
-    if (has_pc) { LE((k_cgdot<T, true>), N, g, gprev, z, mask, scal, part, ens); }
-    else { LE((k_cgdot<T, false>), N, g, gprev, z, mask, scal, part, ens); }
+    const bool v4 = vec4_ok(N, ens, g, gprev, has_pc ? z : nullptr, mask);
+    if (has_pc && v4) { LE((k_cgdot<T, true, 4>), N, g, gprev, z, mask, scal, part, ens); }
+    else if (has_pc) { LE((k_cgdot<T, true, 1>), N, g, gprev, z, mask, scal, part, ens); }
+    else if (v4) { LE((k_cgdot<T, false, 4>), N, g, gprev, z, mask, scal, part, ens); }
+    else { LE((k_cgdot<T, false, 1>), N, g, gprev, z, mask, scal, part, ens); }
 }
 template <typename T>
 void VecOps<T>::tupdate(std::int64_t N, const T* g, T* gprev, const T* z,
                         const std::uint8_t* mask, T* t, const double* scal, bool has_pc,
                         hipStream_t s, const Ens& ens) {
-    if (has_pc) { LES((k_tupdate<T, true>), N, g, gprev, z, mask, t, scal, ens); }
-    else { LES((k_tupdate<T, false>), N, g, gprev, z, mask, t, scal, ens); }
+    const bool v4 = vec4_ok(N, ens, g, gprev, has_pc ? z : nullptr, mask, t);
+    if (has_pc && v4) { LES((k_tupdate<T, true, 4>), N, g, gprev, z, mask, t, scal, ens); }
+    else if (has_pc) { LES((k_tupdate<T, true, 1>), N, g, gprev, z, mask, t, scal, ens); }
+    else if (v4) { LES((k_tupdate<T, false, 4>), N, g, gprev, z, mask, t, scal, ens); }
+    else { LES((k_tupdate<T, false, 1>), N, g, gprev, z, mask, t, scal, ens); }
 }
 template <typename T>
 void VecOps<T>::line1(std::int64_t N, const T* r, const T* g, const T* t,
                       const std::uint8_t* mask, double* part, int nblk, hipStream_t s,
                       const Ens& ens) {
-    LE(k_line1<T>, N, r, g, t, mask, part, ens);
+    if (vec4_ok(N, ens, r, g, t, mask)) { LE((k_line1<T, 4>), N, r, g, t, mask, part, ens); }
+    else { LE((k_line1<T, 1>), N, r, g, t, mask, part, ens); }
 }
 template <typename T>
 void VecOps<T>::line2(std::int64_t N, const T* r, const T* t, const std::uint8_t* mask,
                       const double* scal, double* part, int nblk, hipStream_t s,
                       const Ens& ens) {
-    LE(k_line2<T>, N, r, t, mask, scal, part, ens);
+    if (vec4_ok(N, ens, r, t, mask)) { LE((k_line2<T, 4>), N, r, t, mask, scal, part, ens); }
+    else { LE((k_line2<T, 1>), N, r, t, mask, scal, part, ens); }
 }
 template <typename T>
 void VecOps<T>::update(std::int64_t N, const T* pin, T* pout, const T* t, const T* g,
                        const double* scal, double* part, int nblk, hipStream_t s,
                        const Ens& ens) {
-    LE(k_update<T>, N, pin, pout, t, g, scal, part, ens);
+    if (vec4_ok(N, ens, pin, pout, t, g)) { LE((k_update<T, 4>), N, pin, pout, t, g, scal, part, ens); }
+    else { LE((k_update<T, 1>), N, pin, pout, t, g, scal, part, ens); }
 }
 template <typename T>
 void VecOps<T>::rescale(std::int64_t N, T* p, const double* scal, hipStream_t s,
                         const Ens& ens) {
-    LES(k_rescale<T>, N, p, scal, ens);
+    if (vec4_ok(N, ens, p)) { LES((k_rescale<T, 4>), N, p, scal, ens); }
+    else { LES((k_rescale<T, 1>), N, p, scal, ens); }
 }
 template <typename T>
 void VecOps<T>::copy_if(std::int64_t N, const T* x, T* y, const double* scal, int flag,
                         hipStream_t s, const Ens& ens) {
-    LES(k_copy_if<T>, N, x, y, scal, flag, ens);
+    if (vec4_ok(N, ens, x, y)) { LES((k_copy_if<T, 4>), N, x, y, scal, flag, ens); }
+    else { LES((k_copy_if<T, 1>), N, x, y, scal, flag, ens); }
 }
 template <typename T>
 void VecOps<T>::objective(std::int64_t N, const T* p, const T* u, const T* g0s,
