@@ -55,7 +55,8 @@ class NestedParams(C.Structure):
                 ("record_error_history", C.c_int), ("active_set", C.c_int),
                 ("active_delta", C.c_double), ("active_halo", C.c_int),
                 ("active_max_rounds", C.c_int), ("active_all_levels", C.c_int),
-                ("active_occupancy_max", C.c_double), ("reuse_operators", C.c_int)]
+                ("active_occupancy_max", C.c_double), ("reuse_operators", C.c_int),
+                ("graph_max_points", C.c_int64)]
 
 
 class ResultInfo(C.Structure):

@@ -311,7 +311,7 @@ def _nested_params(coarsest, q, leaf_side, precond, precond_engine, precond_radi
                    precision, float_floor, allow_tolerance_relaxation, light_result, backend,
                    record_error_history, active_set, active_delta, active_halo,
                    active_max_rounds, active_all_levels, active_occupancy_max,
-                   reuse_operators) -> _capi.NestedParams:
+                   reuse_operators, graph_max_points=1024 * 1024) -> _capi.NestedParams:
     if precond_engine not in _PRECOND_ENGINES:
         raise ValueError("precond_engine must be 'auto', 'stencil' or 'fft'")
     if precision not in _PRECISIONS:
@@ -340,6 +340,7 @@ def _nested_params(coarsest, q, leaf_side, precond, precond_engine, precond_radi
     p.active_all_levels = int(bool(active_all_levels))
     p.active_occupancy_max = float(active_occupancy_max)
     p.reuse_operators = int(bool(reuse_operators))
+    p.graph_max_points = int(graph_max_points)
     return p
 
 
@@ -351,18 +352,21 @@ def solve_nested(grid_size, gap, p_nominal, domain_size=1.0, E_star=1.0, coarses
                  backend="h2", record_error_history=False, active_set=False,
                  active_delta=0.05, active_halo=2, active_max_rounds=5,
                  active_all_levels=True, active_occupancy_max=0.4,
-                 reuse_operators=False) -> ContactResult:
+                 reuse_operators=False, graph_max_points=1024 * 1024) -> ContactResult:
     """Nested-grid (cascadic) solve (python/bindings.cpp:362-432, defaults :836-853).
 
-    ``reuse_operators`` (not in the reference) keeps each level's built
-    operator in a process cache so repeated solves skip the table build."""
+    Not in the reference: ``reuse_operators`` keeps each level's built
+    operator in a process cache so repeated solves skip the table build;
+    levels of at most ``graph_max_points`` points start on the device loop
+    (one captured iteration replayed as a hipGraph; 0 launches every kernel,
+    with bit-identical results)."""
     grid_size = int(grid_size)
     p = _nested_params(coarsest, q, leaf_side, precond, precond_engine, precond_radius,
                        precond_occupancy_max, precond_cost_gate, coarse_tol,
                        single_precision, precision, float_floor, allow_tolerance_relaxation,
                        light_result, backend, record_error_history, active_set, active_delta,
                        active_halo, active_max_rounds, active_all_levels,
-                       active_occupancy_max, reuse_operators)
+                       active_occupancy_max, reuse_operators, graph_max_points)
     g = _flat(gap, grid_size * grid_size)
     out = C.c_void_p()
     check(lib().aspher_solve_nested(grid_size, dptr(g), float(p_nominal), float(domain_size),
@@ -380,7 +384,7 @@ def solve_nested_device(grid_size, gap_ptr: int, p_nominal, tol=1e-8, max_iter=2
                 allow_tolerance_relaxation=False, light_result=False, backend="h2",
                 record_error_history=False, active_set=False, active_delta=0.05,
                 active_halo=2, active_max_rounds=5, active_all_levels=True,
-                active_occupancy_max=0.4, reuse_operators=False)
+                active_occupancy_max=0.4, reuse_operators=False, graph_max_points=1024 * 1024)
     unknown = set(kw) - set(args)
     if unknown:
         raise TypeError("unexpected keyword arguments: %s" % sorted(unknown))

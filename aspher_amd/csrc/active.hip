@@ -253,6 +253,7 @@ void active_level(Context& ctx, Operator& op, Precond* pc, const double* glvl, d
         aopt.datum_mode = SolveOptions::Datum::caller;
         aopt.datum = datum;
         aopt.precond_cost_gate = np.precond_cost_gate;
+        aopt.graph_max_points = np.graph_max_points;
         r = Result();
         solve_impl<Real>(ctx, op, g0r.p, static_cast<Real>(p_bar), aopt, pc, p0.p, r);
         p0_valid = false;
@@ -316,6 +317,7 @@ void active_level(Context& ctx, Operator& op, Precond* pc, const double* glvl, d
         fopt.g_ref = g_ref;
         fopt.datum = datum;
         fopt.precond_cost_gate = np.precond_cost_gate;
+        fopt.graph_max_points = np.graph_max_points;
         Result f;
         if constexpr (is_double) {
             fopt.datum_mode = SolveOptions::Datum::solver;

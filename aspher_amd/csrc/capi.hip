@@ -99,6 +99,7 @@ NestedParams to_params(const aspher_nested_params* p) {
     np.active_max_rounds = p->active_max_rounds;
     np.active_all_levels = p->active_all_levels != 0;
     np.active_occupancy_max = p->active_occupancy_max;
+    np.graph_max_points = p->graph_max_points;
     return np;
 }
 
@@ -539,6 +540,7 @@ void aspher_nested_params_default(aspher_nested_params* p) {
     p->active_all_levels = 1;
     p->active_occupancy_max = 0.4;
     p->reuse_operators = 0;
+    p->graph_max_points = 1024LL * 1024LL;
 }
 
 int aspher_solve_nested(int grid_size, const double* gap, double p_nominal, double domain_size,

@@ -271,21 +271,23 @@ SpectralPlan<float>& Operator::plan<float>(hipStream_t s) {
     return *plan32_;
 }
 template <typename T>
-void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev) {
+void Operator::apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev, const SpecBatch* gate) {
     if (!built_) throw std::logic_error("FFTOperator::matvec: build() not called");
     if (kind_ == Kind::free_space) {
         SpectralPlan<T>& pl = plan<T>(s);
         int kp = 0;
         const T* kh = table_for<T>(pl, s, kp);
         pl.convolve(x, Ns_, y, Ns_, Symbol::table, kh, kp, 0.0, s, RowIn::plain, RowOut::plain,
-                    nullptr, nullptr, 0, nullptr, ev);
+                    nullptr, nullptr, 0, nullptr, ev, pl.register_blocked() ? gate : nullptr);
     } else if (kind_ == Kind::periodic) {
         // q = 2 pi k/L, u_hat = 2 p_hat/(E*|q|), unnormalised pair -> 1/Ns^2
         // (src/periodic_fft_operator.cpp:11-24)
         const double pi = std::acos(-1.0);
         const double sc = L_ / (pi * E_ * static_cast<double>(Ns_) * Ns_);
-        plan<T>(s).convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s, RowIn::plain,
-                           RowOut::plain, nullptr, nullptr, 0, nullptr, ev);
+        SpectralPlan<T>& pl = plan<T>(s);
+        pl.convolve(x, Ns_, y, Ns_, Symbol::periodic, nullptr, 0, sc, s, RowIn::plain,
+                    RowOut::plain, nullptr, nullptr, 0, nullptr, ev,
+                    pl.register_blocked() ? gate : nullptr);
     } else {
         if (ev) throw std::invalid_argument("stage timing needs a spectral backend");
         const std::int64_t N = (std::int64_t)Ns_ * Ns_;
@@ -413,6 +415,18 @@ const T* Operator::table_for(const SpectralPlan<T>& pl, hipStream_t s, int& pitc
 template const double* Operator::table_for<double>(const SpectralPlan<double>&, hipStream_t,
                                                    int&);
 template const float* Operator::table_for<float>(const SpectralPlan<float>&, hipStream_t, int&);
+
+template <typename T>
+void Operator::apply_gated(const T* x, T* y, hipStream_t s, const double* gscal, int gate) {
+    SpecBatch sb;
+    sb.count = 1;
+    sb.gscal = gscal;
+    sb.gate = gate;
+    apply_t<T>(x, y, s, nullptr, &sb);
+}
+template void Operator::apply_gated<double>(const double*, double*, hipStream_t, const double*,
+                                            int);
+template void Operator::apply_gated<float>(const float*, float*, hipStream_t, const double*, int);
 
 void Operator::apply(const double* x, double* y, hipStream_t s) { apply_t<double>(x, y, s, nullptr); }
 void Operator::apply(const float* x, float* y, hipStream_t s) { apply_t<float>(x, y, s, nullptr); }
@@ -566,33 +580,49 @@ Precond::Precond(Kind kind, int Ns, int radius) : kind_(kind), Ns_(Ns), R_(radiu
 
 template <typename T>
 void Precond::apply(const T* g, const std::uint8_t* mask, T* z, double* scal, double* part,
-                    hipStream_t s) {
+                    hipStream_t s, int gate) {
     static const int dz[1] = {S_ZSUM};
+    // gate >= 0: every launch is skipped on the device while scal[gate] is 0
+    Ens eg;
+    eg.gscal = scal;
+    eg.gate = gate;
     if (kind_ == Kind::stencil) {
         const int nblk = vec_blocks((std::int64_t)Ns_ * Ns_);
         VecOps<T>::stencil(Ns_, g, mask, scal, tapd_.p, tapw_.p, (int)w_.size(), R_, z, part,
-                           nblk, s);
-        finalize<T>(part, nblk, 1, kSum1, dz, scal, STEP_ZMEAN, s);
+                           nblk, s, eg);
+        finalize<T>(part, nblk, 1, kSum1, dz, scal, STEP_ZMEAN, s, eg);
     } else if (kind_ == Kind::fourier) {
+        prepare<T>();
         SpectralPlan<T>* pl;
-        if constexpr (std::is_same_v<T, double>) {
-            if (!plan64_) plan64_ = std::make_unique<SpectralPlan<double>>(Ns_, Ns_, Ns_, Ns_, Ns_);
-            pl = plan64_.get();
-        } else {
-            if (!plan32_) plan32_ = std::make_unique<SpectralPlan<float>>(Ns_, Ns_, Ns_, Ns_, Ns_);
-            pl = plan32_.get();
-        }
+        if constexpr (std::is_same_v<T, double>) pl = plan64_.get();
+        else pl = plan32_.get();
         if (pl->row_out_blocks() > kMaxPartRows)
             throw std::invalid_argument("grid too large for the preconditioner reduction");
+        SpecBatch sb;
+        sb.gscal = scal;
+        sb.gate = gate;
         pl->convolve(g, Ns_, z, Ns_, Symbol::precond, nullptr, 0, (double)norm_, s,
-                     RowIn::masked, RowOut::masked, mask, scal, S_GMEAN, part);
-        finalize<T>(part, pl->row_out_blocks(), 1, kSum1, dz, scal, STEP_ZMEAN, s);
+                     RowIn::masked, RowOut::masked, mask, scal, S_GMEAN, part, nullptr,
+                     gate >= 0 && pl->register_blocked() ? &sb : nullptr);
+        finalize<T>(part, pl->row_out_blocks(), 1, kSum1, dz, scal, STEP_ZMEAN, s, eg);
     }
 }
 
+template <typename T>
+void Precond::prepare() {
+    if (kind_ != Kind::fourier) return;
+    if constexpr (std::is_same_v<T, double>) {
+        if (!plan64_) plan64_ = std::make_unique<SpectralPlan<double>>(Ns_, Ns_, Ns_, Ns_, Ns_);
+    } else {
+        if (!plan32_) plan32_ = std::make_unique<SpectralPlan<float>>(Ns_, Ns_, Ns_, Ns_, Ns_);
+    }
+}
+template void Precond::prepare<double>();
+template void Precond::prepare<float>();
+
 template void Precond::apply<double>(const double*, const std::uint8_t*, double*, double*,
-                                     double*, hipStream_t);
+                                     double*, hipStream_t, int);
 template void Precond::apply<float>(const float*, const std::uint8_t*, float*, double*,
-                                    double*, hipStream_t);
+                                    double*, hipStream_t, int);
 
 } // namespace aspher

@@ -66,6 +66,10 @@ struct SolveOptions {
     double g_ref = 0.0;
     enum class Datum { automatic, solver, caller } datum_mode = Datum::automatic;
     double datum = 0.0;
+    // grids of at most this many points replay each iteration's two enqueue
+    // sequences (operator apply + evaluation; CG step) from captured
+    // hipGraphs instead of launching every kernel (0: never)
+    long long graph_max_points = 1024LL * 1024LL;
 };
 
 struct Stage {
@@ -135,6 +139,11 @@ public:
     // same, recording 4 events around the three spectral launches
     void apply_timed(const double* x, double* y, hipStream_t s, hipEvent_t* ev);
     void apply_timed(const float* x, float* y, hipStream_t s, hipEvent_t* ev);
+    // same, skipped on the device while the device scalar gscal[gate] is 0
+    // (register-blocked spectral plans; other plans and the dense kind apply
+    // unconditionally)
+    template <typename T>
+    void apply_gated(const T* x, T* y, hipStream_t s, const double* gscal, int gate);
 
     // y = S x (spectral kinds) with the intermediate spectra in the caller's
     // workspace `ws` (ws_bytes bytes) instead of the plan's own. Allocates
@@ -166,7 +175,9 @@ public:
     const T* table_for(const SpectralPlan<T>& pl, hipStream_t s, int& pitch);
 
 private:
-    template <typename T> void apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev);
+    template <typename T>
+    void apply_t(const T* x, T* y, hipStream_t s, hipEvent_t* ev,
+                 const SpecBatch* gate = nullptr);
     template <typename T> SpectralPlan<T>& plan(hipStream_t s);
 
     Kind kind_;
@@ -239,9 +250,12 @@ public:
     Kind kind() const { return kind_; }
     // z = M^-1 g on the mask (mean-zeroed); uses scal[S_GMEAN] and leaves
     // scal[S_ZMEAN] for the consumer. Returns false for `none`.
+    // gate >= 0: skipped on the device while scal[gate] is 0
     template <typename T>
     void apply(const T* g, const std::uint8_t* mask, T* z, double* scal, double* part,
-               hipStream_t s);
+               hipStream_t s, int gate = -1);
+    // builds what apply<T> would build on its first call (the Fourier plan)
+    template <typename T> void prepare();
     const std::vector<int>& tap_dx() const { return dx_; }
     const std::vector<int>& tap_dy() const { return dy_; }
     const std::vector<double>& tap_w() const { return w_; }
@@ -324,6 +338,7 @@ struct NestedParams {
     int active_max_rounds = 5;
     bool active_all_levels = true;
     double active_occupancy_max = 0.4;
+    long long graph_max_points = 1024LL * 1024LL;   // SolveOptions::graph_max_points
 };
 
 bool stencil_for_level(const std::string& engine, bool precond, double prev_occupancy,

@@ -840,6 +840,54 @@ __device__ void ens_control(double* s) {
     s[S_PENDING] = 1.0;
 }
 
+// Single-solve iteration control after a fresh evaluation (the device loop
+// of solve_impl): the iteration continues on the device only where the host
+// loop (src/contact_solver.cpp:366-407 and the CG branch) would take the plain
+// CG branch, and then does that branch's bookkeeping (best iterate, stall
+// counter, iteration count). Every other branch -- identification after a
+// failed line search, non-finite values, convergence, the stall limit, the
+// last iteration of the budget, fewer than two contact points -- clears
+// S_ACTIVE before any bookkeeping, so the host loop redoes the iteration's
+// decisions from the state of the previous one.
+__device__ void single_control(double* s) {
+    const double fw = s[S_FW], pen = s[S_PEN];
+    const double merit = fw > pen ? fw : pen;
+    const bool improves = merit < s[S_SEEN] * (1.0 - 1e-4);
+    s[S_SAVEBEST] = 0.0;
+    bool go = false;
+    if (s[S_PENDING] != 0.0 && s[S_OK] == 0.0) {
+        s[S_HOST] = HOST_IDENT;
+    } else if (s[S_FINITE] == 0.0) {
+        s[S_HOST] = HOST_NONFINITE;
+    } else if (s[S_LOAD] <= s[S_LOADTOL] && fw <= s[S_TOL] && pen <= s[S_TOL]) {
+        s[S_STATUS] = 1.0;   // converged
+    } else if (!improves && s[S_STALL] + 1.0 >= s[S_STALL_LIMIT]) {
+        s[S_HOST] = HOST_STALL;
+    } else if (s[S_ITER] + 1.0 >= s[S_MAXIT]) {
+        s[S_HOST] = HOST_MAXIT;
+    } else if (s[S_NC] < 2.0) {
+        s[S_HOST] = HOST_IDENT;
+    } else {
+        go = true;
+    }
+    if (!go) {
+        s[S_ACTIVE] = 0.0;
+        return;
+    }
+    if (s[S_KEEPBEST] != 0.0 && merit < s[S_BEST]) {
+        s[S_BEST] = merit;
+        s[S_SAVEBEST] = 1.0;
+    }
+    if (improves) {
+        s[S_SEEN] = merit;
+        s[S_STALL] = 0.0;
+    } else {
+        s[S_STALL] += 1.0;
+    }
+    s[S_ITER] += 1.0;
+    s[S_PENDING] = 1.0;
+}
+
 template <typename T>
 __device__ void run_step(int step, double* s) {
     const double nc = s[S_NC];
@@ -852,7 +900,8 @@ __device__ void run_step(int step, double* s) {
             break;
         }
         case STEP_EVAL2:
-        case STEP_B_EVAL2: {
+        case STEP_B_EVAL2:
+        case STEP_S_EVAL2: {
             double a = s[S_ALPHA], gm = s[S_GMIN];
             if (nc > 0.0) {  // refine_contact_mean (src/contact_solver.cpp:94-99)
                 const double delta = s[S_GCS] / nc;
@@ -870,6 +919,7 @@ __device__ void run_step(int step, double* s) {
                               ? 1.0 : 0.0;
             s[S_GMEAN] = nc > 0.0 ? s[S_GCS] / nc : 0.0;
             if (step == STEP_B_EVAL2) ens_control(s);
+            if (step == STEP_S_EVAL2) single_control(s);
             break;
         }
         case STEP_ZMEAN:

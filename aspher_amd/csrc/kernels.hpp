@@ -43,6 +43,9 @@ enum Step : int {
     // STEP_EVAL2 followed by the ensemble's convergence / stall / best-iterate
     // control, which clears S_ACTIVE when the instance leaves the loop
     STEP_B_EVAL2,
+    // STEP_EVAL2 followed by the single solve's device control, which
+    // continues only into the plain CG branch of the host loop
+    STEP_S_EVAL2,
 };
 
 template <typename T> struct VecOps {

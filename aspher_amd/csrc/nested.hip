@@ -310,6 +310,7 @@ void solve_nested(int Ns, double L, double E_star, const double* gap_dev, double
         // FP64 stages subtract it on the fly
         o.datum_mode = row.fp32 ? SolveOptions::Datum::caller : SolveOptions::Datum::solver;
         o.precond_cost_gate = np.precond_cost_gate;
+        o.graph_max_points = np.graph_max_points;
         return o;
     };
 

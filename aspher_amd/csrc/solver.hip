@@ -7,11 +7,13 @@
 #include "timer.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <limits>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <type_traits>
 
@@ -50,6 +52,7 @@ Diag read_diag(const double* h) {
     d.finite = h[S_FINITE] != 0.0;
     return d;
 }
+
 
 } // namespace
 
@@ -331,6 +334,147 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     bool pending = false;
     Diag d;
 
+    // ── device-controlled iterations (small grids) ──────────────────────────
+    // Where the kernels are short, the host round trip of every iteration
+    // (read the diagnostics, decide, enqueue) sets the pace. There the loop
+    // starts on the device: one iteration -- apply, evaluation, device
+    // control (single_control), best-iterate copy, CG step -- is captured as
+    // a hipGraph and replayed, the host reading the scalars only every
+    // kDevPoll replays. Every kernel of the replay is gated on S_ACTIVE, so
+    // replays after the control has stopped are empty. Buffers are fixed
+    // here: the update is in place, g_prev is a copy and the best iterate a
+    // copy into its own buffer; the arithmetic is that of the host loop.
+    // When the control stops on anything but convergence, the host loop
+    // takes over at that iteration (its evaluation done) with the device's
+    // bookkeeping, and decides it exactly as it would have.
+    int it0 = 0;
+    bool resumed = false, device_converged = false;
+    const bool device_loop = opt.graph_max_points > 0 && N <= opt.graph_max_points &&
+                             speculate && !opt.record_history && opt.max_iter > 1 &&
+                             fault_mode() == FAULT_NONE;
+    if (device_loop) {
+        constexpr int kDevPoll = 8;
+        op.prepare<Real>(s);   // nothing may be allocated while capturing
+        if (pc) pc->prepare<Real>();
+        {
+            const int idx[8] = {S_ACTIVE, S_STATUS, S_HOST, S_PENDING, S_ITER, S_MAXIT, S_TOL,
+                                S_LOADTOL};
+            const double val[8] = {1.0, 0.0, 0.0, 0.0, 0.0, (double)opt.max_iter, tol, load_tol};
+            set_scal(scal, 8, idx, val, s);
+            const int idx2[6] = {S_BEST, S_SAVEBEST, S_SEEN, S_STALL, S_STALL_LIMIT, S_KEEPBEST};
+            const double val2[6] = {1e300, 0.0, 1e300, 0.0, (double)stall_limit,
+                                    opt.keep_best ? 1.0 : 0.0};
+            set_scal(scal, 6, idx2, val2, s);
+            const int idx3[1] = {S_OK};
+            const double val3[1] = {1.0};
+            set_scal(scal, 1, idx3, val3, s);
+        }
+        Ens eg;   // single solve, every launch gated on S_ACTIVE
+        eg.gscal = scal;
+        eg.gate = S_ACTIVE;
+        Real* PB = opt.keep_best ? bbest.p : nullptr;
+        const bool has_pc = precond_on;
+        auto iteration = [&](bool timed) {
+            static const int o1[3] = {RSUM, RSUM, RMIN};
+            static const int d1[3] = {S_GSUM, S_NC, S_VMIN};
+            static const int d2[5] = {S_PV, S_GCS, S_E, S_GC, S_PSUM};
+            static const int dg[3] = {S_G, S_GPR, S_GG};
+            static const int dl1[2] = {S_RSUM, S_NUM};
+            static const int dl2[1] = {S_DEN};
+            static const int ou[2] = {RMAX, RSUM};
+            static const int du[2] = {S_CHANGED, S_TOTAL};
+            if (timed) tm_mv.start(s);
+            op.apply_gated(P, U, s, scal, S_ACTIVE);
+            if (timed) tm_mv.stop(s);
+            V::eval1(N, U, g0s, P, G, part, nblk, s, eg);
+            finalize<Real>(part, nblk, 3, o1, d1, scal, STEP_EVAL1, s, eg);
+            V::eval2(N, P, G, mask, scal, part, nblk, s, eg);
+            finalize<Real>(part, nblk, 5, kS5, d2, scal, STEP_S_EVAL2, s, eg);
+            if (PB) V::copy_if(N, P, PB, scal, S_SAVEBEST, s, eg);
+            if (has_pc) {
+                if (timed) tm_pc.start(s);
+                pc->apply<Real>(G, mask, Z, scal, part, s, S_ACTIVE);
+                if (timed) tm_pc.stop(s);
+            }
+            V::cgdot(N, G, GP, Z, mask, scal, has_pc, part, nblk, s, eg);
+            finalize<Real>(part, nblk, 3, kS3, dg, scal, STEP_CGDOT, s, eg);
+            V::tupdate(N, G, GP, Z, mask, T_, scal, has_pc, s, eg);
+            if (timed) tm_mv.start(s);
+            op.apply_gated(T_, R, s, scal, S_ACTIVE);
+            if (timed) tm_mv.stop(s);
+            V::line1(N, R, G, T_, mask, part, nblk, s, eg);
+            finalize<Real>(part, nblk, 2, kS2, dl1, scal, STEP_LINE1, s, eg);
+            V::line2(N, R, T_, mask, scal, part, nblk, s, eg);
+            finalize<Real>(part, nblk, 1, kS1, dl2, scal, STEP_LINE2, s, eg);
+            V::update(N, P, P, T_, G, scal, part, nblk, s, eg);
+            finalize<Real>(part, nblk, 2, ou, du, scal, STEP_UPDATE, s, eg);
+            V::rescale(N, P, scal, s, eg);
+        };
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        auto release = [&]() {
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+        };
+        try {
+            // the first iteration eagerly: its applies are the timers' samples
+            iteration(true);
+            sync_read();
+            if (hs[S_ACTIVE] != 0.0) {
+                ASPHER_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    iteration(false);
+                } catch (...) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    throw;
+                }
+                ASPHER_HIP(hipStreamEndCapture(s, &graph));
+                ASPHER_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                // each replay either runs an iteration or (stopped) nothing;
+                // the control stops by the iteration budget at the latest
+                for (long long r = 0; r <= opt.max_iter && hs[S_ACTIVE] != 0.0;) {
+                    for (int k = 0; k < kDevPoll; ++k, ++r) ASPHER_HIP(hipGraphLaunch(gexec, s));
+                    sync_read();
+                }
+                if (hs[S_ACTIVE] != 0.0)
+                    throw std::logic_error("solve_contact: device iterations overran the budget");
+            }
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+        // iterations 0 .. it0-1 took the CG branch on the device; iteration
+        // it0 was evaluated (one apply) and stopped there
+        it0 = (int)hs[S_ITER];
+        mv += 2LL * it0 + 1;
+        if (has_pc) pcount += it0;
+        for (long long k = 0; k < 2LL * it0 + 1 - 2; ++k) tm_mv.skip();
+        if (has_pc)
+            for (long long k = 0; k < (long long)it0 - 1; ++k) tm_pc.skip();
+        d = read_diag(hs);
+        if (hs[S_STATUS] == 1.0) {
+            status = Status::converged;
+            u_fresh = true;
+            device_converged = true;   // no host iterations
+        } else {
+            resumed = true;
+            pending = hs[S_PENDING] != 0.0;
+            if (opt.keep_best && hs[S_BEST] < 1e300) {
+                // the best iterate is the copy in its own buffer
+                best_merit = hs[S_BEST];
+                best_idx = 2;
+                have_best = true;
+            }
+            if (hs[S_SEEN] < 1e300) seen_merit = hs[S_SEEN];
+            stall = (int)hs[S_STALL];
+        }
+    }
+
     // g_prev by rotation: the gradient buffer of the previous iteration
     // becomes g_prev and the next evaluation writes the other one. It equals
     // the reference's copy (g_prev = g in the CG step) whenever g_prev is
@@ -338,24 +482,30 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     // without support change, i.e. when the previous iteration took the CG
     // branch; after a restart (delta = 0) its value does not enter. Both
     // buffers start at zero, so it is always finite.
-    int it = 0;
-    for (it = 0; it < opt.max_iter; ++it) {
-        if (!u_fresh) apply(P, U);
-        u_fresh = false;
-        std::swap(G, GP);
-        evaluate();
-        if (speculate) {
-            ctx.copy_scal_async();
-            spec_mv0 = mv;
-            spec_pc0 = pcount;
-            spec_o = enqueue_cg();
-            ctx.wait_scal();
-            tm_mv.drain();
-            tm_pc.drain();
+    int it = it0;
+    for (; !device_converged && it < opt.max_iter; ++it) {
+        if (resumed) {
+            // the device loop stopped here: this iteration is evaluated and
+            // its diagnostics are on the host
+            resumed = false;
         } else {
-            sync_read();
+            if (!u_fresh) apply(P, U);
+            u_fresh = false;
+            std::swap(G, GP);
+            evaluate();
+            if (speculate) {
+                ctx.copy_scal_async();
+                spec_mv0 = mv;
+                spec_pc0 = pcount;
+                spec_o = enqueue_cg();
+                ctx.wait_scal();
+                tm_mv.drain();
+                tm_pc.drain();
+            } else {
+                sync_read();
+            }
+            d = read_diag(hs);
         }
-        d = read_diag(hs);
         if (pending) {
             pending = false;
             if (hs[S_OK] == 0.0) {

@@ -82,9 +82,11 @@ def profiled_issue(kernel):
     return None
 
 
-def run_solve(hc, Ns, gptr, precision, reuse, float_floor, active=False):
+def run_solve(hc, Ns, gptr, precision, reuse, float_floor, active=False, graph=True):
     kw = dict(coarsest=64, backend="fft", precision=precision, light_result=False,
               reuse_operators=reuse, active_set=active)
+    if not graph:
+        kw["graph_max_points"] = 0
     if float_floor is not None:
         kw["float_floor"] = float_floor
     return hc.solve_nested_device(Ns, gptr, 0.005, tol=1e-6, **kw)
@@ -233,16 +235,16 @@ def ensemble_problems(hc, n=64, seed=0):
     return probs
 
 
-def run_ensemble(hc, DeviceArray, steps, tol=1e-5):
+def run_ensemble(hc, DeviceArray, steps, tol=1e-5, graph=True):
     probs = ensemble_problems(hc)
     dev = [DeviceArray.from_numpy(p["gap"], np.float64) for p in probs]
     dprobs = [dict(p, gap_ptr=d.ptr) for p, d in zip(probs, dev)]
     for p in dprobs:
         p.pop("gap")
-    hc.solve_ensemble_device(dprobs, tol=tol)             # warm-up: builds + autotuning
+    hc.solve_ensemble_device(dprobs, tol=tol, use_graph=graph)   # warm-up: builds + autotuning
     t0 = time.perf_counter()
     for _ in range(steps):
-        res, st = hc.solve_ensemble_device(dprobs, tol=tol)
+        res, st = hc.solve_ensemble_device(dprobs, tol=tol, use_graph=graph)
     dt = (time.perf_counter() - t0) / steps
     conv = sum(r.status == "converged" for r in res)
     lat = {}
@@ -293,6 +295,9 @@ def main():
                     help="only the matvec timing (for profiler runs)")
     ap.add_argument("--solve-once", choices=("mixed", "fp64"),
                     help="one cached-operator solve of the given precision (for profiler runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel of --solve-once / --ensemble-only (no hipGraph "
+                         "replay): rocprofv3 kernel tracing faults on replayed graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the 64-instance batched workload")
@@ -313,16 +318,18 @@ def main():
     assert hc.device_count() >= 1, "no GPU visible"
 
     if args.ensemble_only:
-        print(json.dumps({"ensemble": run_ensemble(hc, DeviceArray, args.steps)}))
+        print(json.dumps({"ensemble": run_ensemble(hc, DeviceArray, args.steps,
+                                                   graph=not args.no_graph)}))
         return
 
     if args.solve_once:
         gap = rough_gap(Ns, H=0.8, k_low=4, k_high=64, rms=0.002, seed=0)
         g = DeviceArray.from_numpy(gap, np.float64)
         prec = "float_then_double" if args.solve_once == "mixed" else "double"
-        run_solve(hc, Ns, g.ptr, prec, True, args.float_floor)      # builds the operators
+        gr = not args.no_graph
+        run_solve(hc, Ns, g.ptr, prec, True, args.float_floor, graph=gr)   # builds the operators
         t0 = time.perf_counter()
-        r = run_solve(hc, Ns, g.ptr, prec, True, args.float_floor)
+        r = run_solve(hc, Ns, g.ptr, prec, True, args.float_floor, graph=gr)
         log("[bench] %s solve %.3f s %s" % (args.solve_once, time.perf_counter() - t0, r))
         return
 

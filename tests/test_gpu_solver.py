@@ -319,3 +319,50 @@ def test_overlapped_cg_step_matches_synchronous(hc, backend, precision, tol, max
     np.testing.assert_array_equal(a.pressure, b.pressure)
     np.testing.assert_array_equal(a.gap, b.gap)
     assert [s["iterations"] for s in a.stage_stats] == [s["iterations"] for s in b.stage_stats]
+
+
+@pytest.mark.parametrize("backend,precision,tol,max_iter", [
+    ("fft", "double", 1e-10, 20000),
+    ("fft", "float_then_double", 1e-8, 20000),
+    ("periodic", "float", 1e-6, 20000),
+    ("periodic", "double", 1e-25, 300),     # stagnation: stall, forced identification
+])
+def test_graph_replay_matches_launched_kernels(hc, backend, precision, tol, max_iter):
+    """Levels of at most graph_max_points points replay each CG iteration's
+    two enqueue sequences from captured hipGraphs; each replay runs between
+    the solver's own eager work and host synchronisations (identification
+    steps, the final evaluation). With graph_max_points = 0 every kernel is
+    launched: every result must be bit-identical, run after run."""
+    import ctypes as C
+    from aspher_amd import api, _capi
+    Ns = 256
+    g = hc.surfaces.rough_gap(Ns, H=0.8, seed=5) if backend == "fft" else _periodic_rough(Ns)
+    kw = dict(coarsest=32, q=6, leaf_side=8, precond=True, precond_engine="auto",
+              precond_radius=2, precond_occupancy_max=0.4, precond_cost_gate=0.0,
+              coarse_tol=1e-4, single_precision=False, precision=precision, float_floor=2e-6,
+              allow_tolerance_relaxation=True, light_result=False, backend=backend,
+              record_error_history=False, active_set=False, active_delta=0.05, active_halo=2,
+              active_max_rounds=5, active_all_levels=True, active_occupancy_max=0.4,
+              reuse_operators=False)
+
+    def run(graph_points):
+        p = api._nested_params(**kw)
+        p.graph_max_points = graph_points
+        gg = np.ascontiguousarray(g, dtype=np.float64).ravel()
+        out = C.c_void_p()
+        _capi.check(_capi.lib().aspher_solve_nested(
+            Ns, gg.ctypes.data_as(C.POINTER(C.c_double)), 0.01, 1.0, 1.0, tol, max_iter, 1,
+            C.byref(p), C.byref(out)))
+        return api.ContactResult(out.value)
+
+    ref = run(0)
+    for _ in range(2):
+        a = run(Ns * Ns)
+        assert a.status == ref.status and a.status_reason == ref.status_reason
+        assert a.iterations == ref.iterations and a.matvec_count == ref.matvec_count
+        assert a.precond_count == ref.precond_count
+        assert a.identification_steps == ref.identification_steps
+        np.testing.assert_array_equal(a.pressure, ref.pressure)
+        np.testing.assert_array_equal(a.gap, ref.gap)
+        assert [s["iterations"] for s in a.stage_stats] == \
+            [s["iterations"] for s in ref.stage_stats]
