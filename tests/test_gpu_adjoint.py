@@ -177,3 +177,60 @@ def test_torch_autograd_function(hc):
     # zero off the support, nonzero on it
     pc = p.detach().numpy().ravel() > 0
     assert np.all(dg.numpy().ravel()[~pc] == 0.0) and np.any(dg.numpy().ravel()[pc] != 0.0)
+
+
+def test_torch_autograd_device_tensors(hc):
+    """Device tensors stay in HBM: the forward solve reads the gap through
+    its device pointer, the pressure is copied device to device into the
+    output tensor, and the backward passes dL/dp and dL/dgap as device
+    pointers. Run in a fresh process that imports torch first (torch and the
+    library then share torch's HIP runtime): pressure and both gradients
+    equal the host-array path's bit for bit."""
+    pytest.importorskip("torch")
+    import os
+    import subprocess
+    import sys
+    import textwrap
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = textwrap.dedent("""
+        import sys
+        import numpy as np
+        import torch
+        sys.path.insert(0, %r)
+        assert torch.cuda.is_available()
+        import aspher_amd as hc
+        from aspher_amd import autograd as ag
+        assert ag._one_hip_runtime()
+        calls = []
+        real = ag._solve_device
+        def counted(*a, **k):
+            calls.append(1)
+            return real(*a, **k)
+        ag._solve_device = counted
+        Ns, pbar = 128, 0.005
+        cfg = ag.ContactConfig(grid_size=Ns, backend="fft", tol=1e-12, coarsest=32)
+        gap = hc.surfaces.rough_gap(Ns, seed=0).reshape(Ns, Ns)
+        pt = hc.solve_nested(Ns, hc.surfaces.rough_gap(Ns, seed=1), pbar, coarsest=32,
+                             backend="fft", tol=1e-12).pressure
+        out = {}
+        for dev in ("cpu", "cuda"):
+            g0 = torch.tensor(gap, device=dev, requires_grad=True)
+            pn = torch.tensor(pbar, dtype=torch.float64, device=dev, requires_grad=True)
+            p = ag.contact_pressure(g0, pn, cfg)
+            assert p.device.type == dev and p.dtype == torch.float64
+            loss = 0.5 * ((p - torch.tensor(pt, device=dev)) ** 2).sum()
+            dg, dpn = torch.autograd.grad(loss, (g0, pn))
+            assert dg.device.type == dev
+            out[dev] = (p.detach().cpu().numpy(), dg.cpu().numpy(), float(dpn.cpu()))
+        assert len(calls) == 1, calls
+        assert np.array_equal(out["cpu"][0], out["cuda"][0])
+        assert np.array_equal(out["cpu"][1], out["cuda"][1])
+        assert out["cpu"][2] == out["cuda"][2]
+        assert np.any(out["cuda"][1] != 0.0)
+        print("device autograd ok")
+    """ % root)
+    print("device autograd subprocess", flush=True)
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout[-2000:], r.stderr[-3000:], flush=True)
+    assert r.returncode == 0 and "device autograd ok" in r.stdout
