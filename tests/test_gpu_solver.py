@@ -321,13 +321,15 @@ def test_overlapped_cg_step_matches_synchronous(hc, backend, precision, tol, max
     assert [s["iterations"] for s in a.stage_stats] == [s["iterations"] for s in b.stage_stats]
 
 
-@pytest.mark.parametrize("backend,precision,tol,max_iter", [
-    ("fft", "double", 1e-10, 20000),
-    ("fft", "float_then_double", 1e-8, 20000),
-    ("periodic", "float", 1e-6, 20000),
-    ("periodic", "double", 1e-25, 300),     # stagnation: stall, forced identification
+@pytest.mark.parametrize("backend,precision,tol,max_iter,Ns,coarsest", [
+    ("fft", "double", 1e-10, 20000, 256, 32),
+    ("fft", "float_then_double", 1e-8, 20000, 256, 32),
+    ("periodic", "float", 1e-6, 20000, 256, 32),
+    ("periodic", "double", 1e-25, 300, 256, 32),   # stagnation: stall, forced identification
+    ("fft", "double", 1e-10, 20000, 96, 24),       # generic (non power-of-two) plans, ungated applies
 ])
-def test_graph_replay_matches_launched_kernels(hc, backend, precision, tol, max_iter):
+def test_graph_replay_matches_launched_kernels(hc, backend, precision, tol, max_iter, Ns,
+                                               coarsest):
     """Levels of at most graph_max_points points replay each CG iteration's
     two enqueue sequences from captured hipGraphs; each replay runs between
     the solver's own eager work and host synchronisations (identification
@@ -335,9 +337,8 @@ def test_graph_replay_matches_launched_kernels(hc, backend, precision, tol, max_
     launched: every result must be bit-identical, run after run."""
     import ctypes as C
     from aspher_amd import api, _capi
-    Ns = 256
     g = hc.surfaces.rough_gap(Ns, H=0.8, seed=5) if backend == "fft" else _periodic_rough(Ns)
-    kw = dict(coarsest=32, q=6, leaf_side=8, precond=True, precond_engine="auto",
+    kw = dict(coarsest=coarsest, q=6, leaf_side=8, precond=True, precond_engine="auto",
               precond_radius=2, precond_occupancy_max=0.4, precond_cost_gate=0.0,
               coarse_tol=1e-4, single_precision=False, precision=precision, float_floor=2e-6,
               allow_tolerance_relaxation=True, light_result=False, backend=backend,
