@@ -580,7 +580,7 @@ Precond::Precond(Kind kind, int Ns, int radius) : kind_(kind), Ns_(Ns), R_(radiu
 
 template <typename T>
 void Precond::apply(const T* g, const std::uint8_t* mask, T* z, double* scal, double* part,
-                    hipStream_t s, int gate) {
+                    hipStream_t s, int gate, bool zero_off) {
     static const int dz[1] = {S_ZSUM};
     // gate >= 0: every launch is skipped on the device while scal[gate] is 0
     Ens eg;
@@ -589,7 +589,7 @@ void Precond::apply(const T* g, const std::uint8_t* mask, T* z, double* scal, do
     if (kind_ == Kind::stencil) {
         const int nblk = vec_blocks((std::int64_t)Ns_ * Ns_);
         VecOps<T>::stencil(Ns_, g, mask, scal, tapd_.p, tapw_.p, (int)w_.size(), R_, z, part,
-                           nblk, s, eg);
+                           nblk, s, eg, zero_off);
         finalize<T>(part, nblk, 1, kSum1, dz, scal, STEP_ZMEAN, s, eg);
     } else if (kind_ == Kind::fourier) {
         prepare<T>();
@@ -621,8 +621,8 @@ template void Precond::prepare<double>();
 template void Precond::prepare<float>();
 
 template void Precond::apply<double>(const double*, const std::uint8_t*, double*, double*,
-                                     double*, hipStream_t, int);
+                                     double*, hipStream_t, int, bool);
 template void Precond::apply<float>(const float*, const std::uint8_t*, float*, double*,
-                                    double*, hipStream_t, int);
+                                    double*, hipStream_t, int, bool);
 
 } // namespace aspher

@@ -250,10 +250,12 @@ public:
     Kind kind() const { return kind_; }
     // z = M^-1 g on the mask (mean-zeroed); uses scal[S_GMEAN] and leaves
     // scal[S_ZMEAN] for the consumer. Returns false for `none`.
-    // gate >= 0: skipped on the device while scal[gate] is 0
+    // gate >= 0: skipped on the device while scal[gate] is 0; zero_off =
+    // false: the stencil leaves z unwritten off the mask (callers that read
+    // z only on the mask)
     template <typename T>
     void apply(const T* g, const std::uint8_t* mask, T* z, double* scal, double* part,
-               hipStream_t s, int gate = -1);
+               hipStream_t s, int gate = -1, bool zero_off = true);
     // builds what apply<T> would build on its first call (the Fourier plan)
     template <typename T> void prepare();
     const std::vector<int>& tap_dx() const { return dx_; }

@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256)
 k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask,
           const double* __restrict__ scal, const int* __restrict__ tapd,
           const double* __restrict__ tapw, int ntaps, int Rrt, T* __restrict__ z,
-          double* __restrict__ part, Ens ens) {
+          double* __restrict__ part, Ens ens, int zero_off) {
     __shared__ double red[8];
     extern __shared__ __align__(16) unsigned char smem[];
     double* tile = reinterpret_cast<double*>(smem);
@@ -290,7 +290,9 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
             if (ix >= Ns || iy >= Ns) continue;
             const std::size_t i = (std::size_t)iy * Ns + ix;
             if (!mask[i]) {
-                z[i] = T(0);
+                // z off the mask is zero; a caller that reads z only on the
+                // mask skips these stores
+                if (zero_off) z[i] = T(0);
                 continue;
             }
             double acc = 0.0;
@@ -1387,20 +1389,22 @@ void VecOps<T>::eval2(std::int64_t N, const T* p, T* g, std::uint8_t* mask,
 template <typename T>
 void VecOps<T>::stencil(int Ns, const T* g, const std::uint8_t* mask, const double* scal,
                         const int* tapd, const double* tapw, int ntaps, int R, T* z,
-                        double* part, int nblk, hipStream_t s, const Ens& ens) {
+                        double* part, int nblk, hipStream_t s, const Ens& ens,
+                        bool zero_off) {
     const std::size_t lds = sizeof(double) * (kStX + 2 * R) * (kStY + 2 * R);
     const int grid = ens.work ? ens.nwork : nblk;
+    const int zo = zero_off ? 1 : 0;
     // the unrolled kernels assume the l2 disc's tap count of their radius
     static const int disc_taps[5] = {0, 5, 13, 29, 49};
     switch (R >= 1 && R <= 4 && ntaps == disc_taps[R] ? R : 0) {
 #define ST_CASE(RR) \
         case RR: k_stencil<T, RR><<<grid, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, \
-                                                               ntaps, R, z, part, ens); break;
+                                                               ntaps, R, z, part, ens, zo); break;
         ST_CASE(1) ST_CASE(2) ST_CASE(3) ST_CASE(4)
 #undef ST_CASE
         default:
             k_stencil<T, 0><<<grid, kVecThreads, lds, s>>>(Ns, g, mask, scal, tapd, tapw, ntaps, R,
-                                                          z, part, ens);
+                                                          z, part, ens, zo);
     }
     ASPHER_LAUNCH_CHECK();
 }

@@ -58,10 +58,12 @@ template <typename T> struct VecOps {
                       const double* scal, double* part, int nblk, hipStream_t s,
                       const Ens& ens = Ens{});
     // 13-tap (any radius) masked stencil (uses S_GMEAN): z and partial zsum
+    // zero_off = false: z is left unwritten off the mask (for callers that
+    // read z only on the mask)
     static void stencil(int Ns, const T* g, const std::uint8_t* mask,
                         const double* scal, const int* tapd, const double* tapw,
                         int ntaps, int R, T* z, double* part, int nblk, hipStream_t s,
-                        const Ens& ens = Ens{});
+                        const Ens& ens = Ens{}, bool zero_off = true);
     // {G, G_pr, gg} over the mask; z is (z - zmean) when has_pc, else g
     static void cgdot(std::int64_t N, const T* g, const T* gprev, const T* z,
                       const std::uint8_t* mask, const double* scal, bool has_pc,

@@ -270,7 +270,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
         const bool has_pc = precond_on;
         if (has_pc) {
             tm_pc.start(s);
-            pc->apply<Real>(G, mask, Z, scal, part, s);
+            pc->apply<Real>(G, mask, Z, scal, part, s, -1, false);
             tm_pc.stop(s);
             ++pcount;
             if (opt.precond_cost_gate > 0.0 && pcount >= 1 && mv >= 1) {
@@ -393,7 +393,7 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             if (PB) V::copy_if(N, P, PB, scal, S_SAVEBEST, s, eg);
             if (has_pc) {
                 if (timed) tm_pc.start(s);
-                pc->apply<Real>(G, mask, Z, scal, part, s, S_ACTIVE);
+                pc->apply<Real>(G, mask, Z, scal, part, s, S_ACTIVE, false);
                 if (timed) tm_pc.stop(s);
             }
             V::cgdot(N, G, GP, Z, mask, scal, has_pc, part, nblk, s, eg);
