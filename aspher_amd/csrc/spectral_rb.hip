@@ -132,22 +132,31 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
             lds[rb::lds_ix(Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v)))] = r[q * Nl + v];
     __syncthreads();
     // store: rows of the workgroup fastest, so B consecutive rows of one kx
-    // form one contiguous run of the transposed layout
+    // form one contiguous run of the transposed layout. Element e =
+    // threadIdx.x + i*blockDim.x of the B*(L+1) outputs is row e % B,
+    // frequency e / B; with blockDim.x = B*TPR that is row threadIdx.x % B and
+    // frequency threadIdx.x / B + i*TPR, so no division per element
     const int nb = (nrows - blk * B < B) ? (nrows - blk * B) : B;
-    for (int e = threadIdx.x; e < B * (L + 1); e += blockDim.x) {
-        const int bb = e % B, k = e / B;
-        if (bb >= nb) continue;
+    const int bb = threadIdx.x % B, k0 = threadIdx.x / B;
+    if (bb < nb) {
         const cx<T>* lb = lds_all + bb * LL;
-        const cx<T> Zk = lb[rb::lds_ix(k == L ? 0 : k)];
-        const cx<T> Zc = conjg(lb[rb::lds_ix(k == 0 ? 0 : L - k)]);
-        const cx<T> E = scale(Zk + Zc, T(0.5));
-        const cx<T> D = Zk - Zc;
-        const cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};
-        // W^k = W^(k - k%64) W^(k%64): both factors from a small, cache-hot
-        // subset of the table
-        const cx<double> wd = tw[k & ~63] * tw[k & 63];
-        const cx<T> X = E + cx<T>{static_cast<T>(wd.re), static_cast<T>(wd.im)} * O;
-        AT[(size_t)k * pitchT + blk * B + bb] = X;
+        cx<T>* out = AT + blk * B + bb;
+        constexpr int KI = L / TPR + 1;
+#pragma unroll
+        for (int i = 0; i < KI; ++i) {
+            const int k = k0 + i * TPR;
+            if (k > L) break;
+            const cx<T> Zk = lb[rb::lds_ix(k == L ? 0 : k)];
+            const cx<T> Zc = conjg(lb[rb::lds_ix(k == 0 ? 0 : L - k)]);
+            const cx<T> E = scale(Zk + Zc, T(0.5));
+            const cx<T> D = Zk - Zc;
+            const cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};
+            // W^k = W^(k - k%64) W^(k%64): both factors from a small,
+            // cache-hot subset of the table
+            const cx<double> wd = tw[k & ~63] * tw[k & 63];
+            const cx<T> X = E + cx<T>{static_cast<T>(wd.re), static_cast<T>(wd.im)} * O;
+            out[(size_t)k * pitchT] = X;
+        }
     }
 }
 
@@ -367,19 +376,21 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
     cx<T> xv[J];
     if (in_t) {
         // transposed input BR[k*pitch + row] (pitch = the column pitch): the
-        // workgroup's B rows fastest, so each k is one run of B values
+        // workgroup's B rows fastest, so each k is one run of B values.
+        // Element e is row e % B, frequency e / B; with blockDim.x = B*TPR
+        // that is row threadIdx.x % B, frequency threadIdx.x / B + j*TPR
+        static_assert(J == L / TPR + 1, "one load per frequency step");
+        const int bb = threadIdx.x % B, k0 = threadIdx.x / B;
+        const int rr = blk * B + bb;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int e = threadIdx.x + j * (int)blockDim.x;
-            const int k = e / B, bb = e - k * B;
-            const int rr = blk * B + bb;
-            xv[j] = (e < lim && rr < nrows) ? BR[(size_t)k * pitch + rr] : cx<T>{T(0), T(0)};
+            const int k = k0 + j * TPR;
+            xv[j] = (k <= L && rr < nrows) ? BR[(size_t)k * pitch + rr] : cx<T>{T(0), T(0)};
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int e = threadIdx.x + j * (int)blockDim.x;
-            const int k = e / B, bb = e - k * B;
-            if (e < lim) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+            const int k = k0 + j * TPR;
+            if (k <= L) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
         }
     } else {
 #pragma unroll
