@@ -463,18 +463,18 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
         // The column pass writes its result back into AT, transposed: every
         // store is a contiguous run, where row-major stores of single
         // elements into BR were written back ~2.15x over (measured at
-        // 4096^2 FP32). The inverse row pass then reads runs of
-        // row_batch_inv_ rows per frequency: at least 32 bytes (the forward
-        // pass's write runs), at least the row-major pass's batch, within
-        // the forward pass's LDS budget. Measured at 4096^2: FP32 matvec
-        // 0.326 -> 0.292 ms, FP64 0.541 -> 0.535 ms; 2048^2 FP64 0.137 ->
-        // 0.118 ms.
+        // 4096^2 FP32). Measured at 4096^2: FP32 matvec 0.326 -> 0.292 ms,
+        // FP64 0.541 -> 0.535 ms; 2048^2 FP64 0.137 -> 0.118 ms.
+        // The inverse row pass then reads runs of row_batch_inv_ = 2 rows per
+        // frequency (16-byte runs in FP32, 32-byte in FP64; at least the
+        // row-major pass's batch, within the forward pass's LDS budget): two
+        // 512-thread workgroups are resident per CU, so one's loads overlap
+        // the other's transform. Measured at 4096^2 FP32, inverse rows: 4
+        // rows (one 1024-thread workgroup per CU) 84 us, 2 rows 78 us, 1
+        // row (8-byte runs) 124 us; mixed solve 0.357 -> 0.352 s.
         tcols_ = tune.col_out_t;
         row_batch_inv_ = row_batch_;
-        if (tcols_) {
-            const int run32 = std::max<int>(1, 32 / (int)cs);
-            row_batch_inv_ = std::min(row_batch_fwd_, std::max(row_batch_, run32));
-        }
+        if (tcols_) row_batch_inv_ = std::min(row_batch_fwd_, std::max(row_batch_, 2));
         tw64_ = std::make_unique<Twiddles<double>>(n);
         a_rows_ = static_cast<std::size_t>(std::max(rows_out, 1));
         br_stride_ = (std::int64_t)a_rows_ * pitch;
