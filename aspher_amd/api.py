@@ -311,7 +311,7 @@ def _nested_params(coarsest, q, leaf_side, precond, precond_engine, precond_radi
                    precision, float_floor, allow_tolerance_relaxation, light_result, backend,
                    record_error_history, active_set, active_delta, active_halo,
                    active_max_rounds, active_all_levels, active_occupancy_max,
-                   reuse_operators, graph_max_points=1024 * 1024) -> _capi.NestedParams:
+                   reuse_operators, graph_max_points=0) -> _capi.NestedParams:
     if precond_engine not in _PRECOND_ENGINES:
         raise ValueError("precond_engine must be 'auto', 'stencil' or 'fft'")
     if precision not in _PRECISIONS:
@@ -352,14 +352,14 @@ def solve_nested(grid_size, gap, p_nominal, domain_size=1.0, E_star=1.0, coarses
                  backend="h2", record_error_history=False, active_set=False,
                  active_delta=0.05, active_halo=2, active_max_rounds=5,
                  active_all_levels=True, active_occupancy_max=0.4,
-                 reuse_operators=False, graph_max_points=1024 * 1024) -> ContactResult:
+                 reuse_operators=False, graph_max_points=0) -> ContactResult:
     """Nested-grid (cascadic) solve (python/bindings.cpp:362-432, defaults :836-853).
 
     Not in the reference: ``reuse_operators`` keeps each level's built
     operator in a process cache so repeated solves skip the table build;
     levels of at most ``graph_max_points`` points start on the device loop
-    (one captured iteration replayed as a hipGraph; 0 launches every kernel,
-    with bit-identical results)."""
+    (one captured iteration replayed as a hipGraph; 0, the default, launches
+    every kernel, with bit-identical results)."""
     grid_size = int(grid_size)
     p = _nested_params(coarsest, q, leaf_side, precond, precond_engine, precond_radius,
                        precond_occupancy_max, precond_cost_gate, coarse_tol,
@@ -384,7 +384,7 @@ def solve_nested_device(grid_size, gap_ptr: int, p_nominal, tol=1e-8, max_iter=2
                 allow_tolerance_relaxation=False, light_result=False, backend="h2",
                 record_error_history=False, active_set=False, active_delta=0.05,
                 active_halo=2, active_max_rounds=5, active_all_levels=True,
-                active_occupancy_max=0.4, reuse_operators=False, graph_max_points=1024 * 1024)
+                active_occupancy_max=0.4, reuse_operators=False, graph_max_points=0)
     unknown = set(kw) - set(args)
     if unknown:
         raise TypeError("unexpected keyword arguments: %s" % sorted(unknown))

@@ -540,7 +540,7 @@ void aspher_nested_params_default(aspher_nested_params* p) {
     p->active_all_levels = 1;
     p->active_occupancy_max = 0.4;
     p->reuse_operators = 0;
-    p->graph_max_points = 1024LL * 1024LL;
+    p->graph_max_points = 0;
 }
 
 int aspher_solve_nested(int grid_size, const double* gap, double p_nominal, double domain_size,

@@ -66,10 +66,14 @@ struct SolveOptions {
     double g_ref = 0.0;
     enum class Datum { automatic, solver, caller } datum_mode = Datum::automatic;
     double datum = 0.0;
-    // grids of at most this many points replay each iteration's two enqueue
-    // sequences (operator apply + evaluation; CG step) from captured
-    // hipGraphs instead of launching every kernel (0: never)
-    long long graph_max_points = 1024LL * 1024LL;
+    // grids of at most this many points start on the device-controlled loop
+    // (one captured iteration replayed as a hipGraph) instead of launching
+    // every kernel (0: never, the default). Measured on MI355X it loses at
+    // the iteration counts of the nested levels and of single solves up to
+    // ~100 iterations: capture and instantiation cost ~0.45 ms per solve,
+    // and a replayed iteration of ~25 dependent small kernels saves only
+    // ~20 us at 64^2 and nothing from 256^2 up.
+    long long graph_max_points = 0;
 };
 
 struct Stage {
@@ -340,7 +344,7 @@ struct NestedParams {
     int active_max_rounds = 5;
     bool active_all_levels = true;
     double active_occupancy_max = 0.4;
-    long long graph_max_points = 1024LL * 1024LL;   // SolveOptions::graph_max_points
+    long long graph_max_points = 0;   // SolveOptions::graph_max_points
 };
 
 bool stencil_for_level(const std::string& engine, bool precond, double prev_occupancy,

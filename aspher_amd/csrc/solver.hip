@@ -334,10 +334,12 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
     bool pending = false;
     Diag d;
 
-    // ── device-controlled iterations (small grids) ──────────────────────────
+    // ── device-controlled iterations (small grids, opt-in) ──────────────────
     // Where the kernels are short, the host round trip of every iteration
-    // (read the diagnostics, decide, enqueue) sets the pace. There the loop
-    // starts on the device: one iteration -- apply, evaluation, device
+    // (read the diagnostics, decide, enqueue) sets the pace. On grids of at
+    // most opt.graph_max_points points (0 by default: the capture and
+    // instantiation outweigh the replay savings at the iteration counts of
+    // nested levels) the loop starts on the device: one iteration -- apply, evaluation, device
     // control (single_control), best-iterate copy, CG step -- is captured as
     // a hipGraph and replayed, the host reading the scalars only every
     // kDevPoll replays. Every kernel of the replay is gated on S_ACTIVE, so

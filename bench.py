@@ -375,9 +375,10 @@ def main():
                            args.steps, args.warmup, w_fp64)
     log("[bench]   %.3f s  %s" % (t_fp64, r_fp64))
     log("[bench] solve-only (operators cached)")
-    t_mixed_so, _ = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff),
-                          args.steps, 1)
-    t_fp64_so, _ = timed(lambda: run_solve(hc, Ns, g.ptr, "double", True, ff), args.steps, 1)
+    t_mixed_so, r_mixed_so = timed(
+        lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff), args.steps, 1)
+    t_fp64_so, r_fp64_so = timed(lambda: run_solve(hc, Ns, g.ptr, "double", True, ff),
+                                 args.steps, 1)
     log("[bench]   mixed %.3f s, fp64 %.3f s" % (t_mixed_so, t_fp64_so))
     log("[bench] mixed with active set (operators cached)")
     t_act_so, r_act = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff,
@@ -512,6 +513,11 @@ def main():
                                         for s in r_mixed.stage_stats],
                        "fp64_stages": [(s["name"], s["iterations"])
                                        for s in r_fp64.stage_stats]},
+        # wall seconds per stage of the last cached-operator solve
+        "stage_seconds": {"mixed": [(s["name"], round(s["seconds"], 4))
+                                    for s in r_mixed_so.stage_stats],
+                          "fp64": [(s["name"], round(s["seconds"], 4))
+                                   for s in r_fp64_so.stage_stats]},
         "accuracy": accuracy,
         "active_set": active,
         "matvec": {"fp32_ms": round(mv[32]["ms"], 4), "fp32_gbs": round(mv[32]["gbs"], 1),

@@ -137,7 +137,7 @@ typedef struct {
     double active_occupancy_max;    /* 0.4 */
     int reuse_operators;            /* 0: rebuild every level like the reference;
                                        1: keep built operators in a process cache */
-    int64_t graph_max_points;       /* 1048576: levels of at most this many points
+    int64_t graph_max_points;       /* 0: levels of at most this many points
                                        replay each CG iteration from captured
                                        hipGraphs (0: launch every kernel; same
                                        results either way). No reference
