@@ -219,13 +219,29 @@ __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 4) + 1; }
 // those (<= 4 roundings, ~2.5e-7 relative, the accuracy of an FP32 twiddle
 // table). One table read per group measured faster than reading every power
 // (the table reads, not the arithmetic, bound the stage).
-template <typename T, int NS, bool INV>
+// FP32 twiddle powers of one radix-16 group (f[v] = W^{v idx}, f[0]
+// unused), formed by a forward stage and reused, conjugated, by the same
+// stage of the inverse, which would otherwise form the powers of the
+// conjugated table value again (the same values in exact arithmetic; the
+// compiled sequences differ in FMA contraction, so results move at the
+// FP32 rounding level: 1.1e-7 relative on a 4096^2 free-space matvec).
+struct TwKeep {
+    cx<float> f[16];
+};
+
+// KEEP: 0 = form the powers; 1 = form them and keep them in *keep
+// (forward); 2 = apply the conjugates of *keep (inverse, same group)
+template <typename T, int NS, bool INV, int KEEP = 0>
 __device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<double>* __restrict__ tw,
-                                              int idx, int /*tmask*/) {
+                                              int idx, int /*tmask*/, TwKeep* keep = nullptr) {
     if (idx == 0) return;
-    cx<double> w = tw[idx];
-    if (INV) w.im = -w.im;
-    if constexpr (std::is_same_v<T, float> && NS == 16) {
+    if constexpr (KEEP == 2) {
+        static_assert(std::is_same_v<T, float> && NS == 16, "kept powers: FP32 radix 16");
+#pragma unroll
+        for (int v = 1; v < 16; ++v) a[v] = a[v] * cx<float>{keep->f[v].re, -keep->f[v].im};
+    } else if constexpr (std::is_same_v<T, float> && NS == 16) {
+        cx<double> w = tw[idx];
+        if (INV) w.im = -w.im;
         const cx<double> w2 = w * w, w4 = w2 * w2, w8 = w4 * w4;
         auto f32 = [](cx<double> z) { return cx<float>{(float)z.re, (float)z.im}; };
         cx<float> f[16];
@@ -239,9 +255,16 @@ __device__ __forceinline__ void twiddle_group(cx<T>* a, const cx<double>* __rest
         f[7] = f[4] * f[3];
 #pragma unroll
         for (int v = 9; v < 16; ++v) f[v] = f[8] * f[v - 8];
+        if constexpr (KEEP == 1) {
+#pragma unroll
+            for (int v = 1; v < 16; ++v) keep->f[v] = f[v];
+        }
 #pragma unroll
         for (int v = 1; v < 16; ++v) a[v] = a[v] * f[v];
     } else {
+        static_assert(KEEP == 0, "kept powers: FP32 radix 16");
+        cx<double> w = tw[idx];
+        if (INV) w.im = -w.im;
         cx<double> wv = w;
 #pragma unroll
         for (int v = 1; v < NS; ++v) {
@@ -281,13 +304,18 @@ __device__ __forceinline__ void twiddle_w32(cx<T>* a, int low) {
 // twiddle then DFT) transform on the registers of thread th. PR (stage 0
 // only, where every thread holds one group of 16 at positions v*S_0 + th):
 // 1 = inputs v >= 8 are zero, 2 = only outputs v < 8 are needed.
-template <typename T, int LOGN, int s, bool INV, int PR = 0>
+// KS: the stage whose FP32 twiddle powers the forward keeps in *keep and
+// the inverse reuses (-1: none; see keep_stage)
+template <typename T, int LOGN, int s, bool INV, int PR = 0, int KS = -1>
 __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __restrict__ tw,
-                                      int twmul) {
+                                      int twmul, TwKeep* keep = nullptr) {
     using Sh = Shape<LOGN>;
     constexpr int Ns = Sh::N(s), Ss = Sh::S(s), Gs = Sh::G(s);
     constexpr int tstep = Sh::n / (Ns * Ss);     // table stride of W_{Ns Ss}
+    constexpr int KP = (s == KS) ? (INV ? 2 : 1) : 0;
     static_assert(PR == 0 || (s == 0 && Ns == 16 && Gs == 1), "pruned stage 0 only");
+    static_assert(KP == 0 || (Ns == 16 && Gs == 1 && Ss > 2 && s < Sh::m - 1),
+                  "kept powers: a twiddled radix-16 stage with one group per thread");
 #pragma unroll
     for (int q = 0; q < Gs; ++q) {
         cx<T>* a = r + q * Ns;
@@ -298,7 +326,8 @@ __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __rest
         if constexpr (INV) {
             if constexpr (s < Sh::m - 1) {
                 if constexpr (W32) twiddle_w32<T, true>(a, low);
-                else twiddle_group<T, Ns, true>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
+                else twiddle_group<T, Ns, true, KP>(a, tw, low * tstep * twmul,
+                                                    Sh::n * twmul - 1, keep);
             }
             if constexpr (PR == 2) dft16_hout<T, true>(a);
             else dft<T, Ns, true>(a);
@@ -307,10 +336,19 @@ __device__ __forceinline__ void stage(cx<T>* r, int th, const cx<double>* __rest
             else dft<T, Ns, false>(a);
             if constexpr (s < Sh::m - 1) {
                 if constexpr (W32) twiddle_w32<T, false>(a, low);
-                else twiddle_group<T, Ns, false>(a, tw, low * tstep * twmul, Sh::n * twmul - 1);
+                else twiddle_group<T, Ns, false, KP>(a, tw, low * tstep * twmul,
+                                                     Sh::n * twmul - 1, keep);
             }
         }
     }
+}
+
+// The stage whose twiddle powers a forward-then-inverse kernel can keep:
+// stage 1 where it is a generic twiddled radix-16 stage (n >= 1024).
+template <int LOGN>
+__host__ __device__ constexpr int keep_stage() {
+    using Sh = Shape<LOGN>;
+    return (Sh::m >= 3 && Sh::N(1) == 16 && Sh::G(1) == 1 && Sh::S(1) > 2) ? 1 : -1;
 }
 
 // No-op hook of the exchanges.
@@ -400,27 +438,28 @@ __device__ __forceinline__ void exchange(cx<T>* r, T* lds, int th,
 }
 
 // LDS element type L: cx<T> (one pass per exchange) or T (split passes)
-template <typename T, int LOGN, int s, bool ZIN, typename Hook, typename L>
+template <typename T, int LOGN, int s, bool ZIN, typename Hook, typename L, int KS>
 __device__ __forceinline__ void forward_from(cx<T>* r, L* lds, int th,
                                              const cx<double>* __restrict__ tw, int twmul,
-                                             const Hook& hook) {
-    stage<T, LOGN, s, false, (s == 0 && ZIN) ? 1 : 0>(r, th, tw, twmul);
+                                             const Hook& hook, TwKeep* keep) {
+    stage<T, LOGN, s, false, (s == 0 && ZIN) ? 1 : 0, KS>(r, th, tw, twmul, keep);
     if constexpr (s + 1 < Shape<LOGN>::m) {
         if constexpr (s + 2 == Shape<LOGN>::m) exchange<T, LOGN, s, s + 1>(r, lds, th, hook);
         else exchange<T, LOGN, s, s + 1>(r, lds, th);
-        forward_from<T, LOGN, s + 1, ZIN, Hook, L>(r, lds, th, tw, twmul, hook);
+        forward_from<T, LOGN, s + 1, ZIN, Hook, L, KS>(r, lds, th, tw, twmul, hook, keep);
     } else if constexpr (s == 0) {
         hook();   // single-stage transform: no exchange to hide the loads in
     }
 }
 
-template <typename T, int LOGN, int s, bool HOUT, typename L>
+template <typename T, int LOGN, int s, bool HOUT, typename L, int KS>
 __device__ __forceinline__ void inverse_from(cx<T>* r, L* lds, int th,
-                                             const cx<double>* __restrict__ tw, int twmul) {
-    stage<T, LOGN, s, true, (s == 0 && HOUT) ? 2 : 0>(r, th, tw, twmul);
+                                             const cx<double>* __restrict__ tw, int twmul,
+                                             TwKeep* keep) {
+    stage<T, LOGN, s, true, (s == 0 && HOUT) ? 2 : 0, KS>(r, th, tw, twmul, keep);
     if constexpr (s > 0) {
         exchange<T, LOGN, s, s - 1>(r, lds, th);
-        inverse_from<T, LOGN, s - 1, HOUT, L>(r, lds, th, tw, twmul);
+        inverse_from<T, LOGN, s - 1, HOUT, L, KS>(r, lds, th, tw, twmul, keep);
     }
 }
 
@@ -428,17 +467,21 @@ __device__ __forceinline__ void inverse_from(cx<T>* r, L* lds, int th,
 // grouping (digit-reversed); inverse: the reverse. ZIN: the upper half of
 // the input (positions >= n/2, i.e. v >= 8 of stage 0) is zero; HOUT: only
 // the lower half of the output is used.
-// `hook` runs once, inside the last exchange (see `exchange`).
-template <typename T, int LOGN, bool ZIN = false, typename Hook = NoHook, typename L = cx<T>>
+// `hook` runs once, inside the last exchange (see `exchange`). KS >= 0: the
+// forward keeps stage KS's twiddle powers in *keep, and an inverse of the
+// same thread with the same KS reuses them.
+template <typename T, int LOGN, bool ZIN = false, typename Hook = NoHook, typename L = cx<T>,
+          int KS = -1>
 __device__ __forceinline__ void forward(cx<T>* r, L* lds, int th,
                                         const cx<double>* __restrict__ tw, int twmul,
-                                        const Hook& hook = Hook{}) {
-    forward_from<T, LOGN, 0, ZIN, Hook, L>(r, lds, th, tw, twmul, hook);
+                                        const Hook& hook = Hook{}, TwKeep* keep = nullptr) {
+    forward_from<T, LOGN, 0, ZIN, Hook, L, KS>(r, lds, th, tw, twmul, hook, keep);
 }
-template <typename T, int LOGN, bool HOUT = false, typename L = cx<T>>
+template <typename T, int LOGN, bool HOUT = false, typename L = cx<T>, int KS = -1>
 __device__ __forceinline__ void inverse(cx<T>* r, L* lds, int th,
-                                        const cx<double>* __restrict__ tw, int twmul) {
-    inverse_from<T, LOGN, Shape<LOGN>::m - 1, HOUT, L>(r, lds, th, tw, twmul);
+                                        const cx<double>* __restrict__ tw, int twmul,
+                                        TwKeep* keep = nullptr) {
+    inverse_from<T, LOGN, Shape<LOGN>::m - 1, HOUT, L, KS>(r, lds, th, tw, twmul, keep);
 }
 
 } // namespace rb

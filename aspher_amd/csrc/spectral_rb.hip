@@ -268,12 +268,20 @@ k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
     // half-size table: mirror-paired threads share a chunk, the upper one
     // reads it reversed (kmap); without a map, one chunk per thread
     int km = th << 1;
+    // FP32 table symbol: the inverse reuses the forward's stage-KS twiddle
+    // powers (15 values more per thread, within the split columns' 128
+    // VGPRs; the computed-symbol passes have no such bound and would lose a
+    // resident workgroup)
+    constexpr int KS = kColsSplit<T, LOGN, SYM> && std::is_same_v<T, float>
+                           ? rb::keep_stage<LOGN>() : -1;
+    rb::TwKeep keep;
     if constexpr (SYM == 0) {
         if (kmap) km = kmap[th];
         const T* kr = ktab + (size_t)(live ? kx : 0) * kp + 16 * (km >> 1);
-        rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1, [&]() { load16(kr, kv); });
+        auto hook = [&]() { load16(kr, kv); };
+        rb::forward<T, LOGN, ZP, decltype(hook), LT, KS>(r, lds, th, tw, 1, hook, &keep);
     } else {
-        rb::forward<T, LOGN, ZP>(r, lds, th, tw, 1);
+        rb::forward<T, LOGN, ZP, rb::NoHook, LT, KS>(r, lds, th, tw, 1, rb::NoHook{}, &keep);
     }
 
     if constexpr (SYM == 3) {
@@ -316,7 +324,7 @@ k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
                     r[q * Nl + v] = scale(r[q * Nl + v], s);
                 }
         }
-        rb::inverse<T, LOGN, ZP>(r, lds, th, tw, 1);
+        rb::inverse<T, LOGN, ZP, LT, KS>(r, lds, th, tw, 1, &keep);
         if (live && out_t) {
             // transposed, in place: the column's rows [0, rows_out) back into
             // its own AT column (every thread's loads of it were consumed
