@@ -455,7 +455,9 @@ def solve_ensemble(problems, tol=1e-5, max_iter=5000, use_pr=True, precision="fl
     ``problems``: list of dicts with ``grid_size``, ``gap`` ((N,) or (Ns, Ns)),
     ``p_nominal`` and optionally ``backend`` ("fft" | "periodic"),
     ``domain_size``, ``E_star``. Returns (list of ContactResult in input
-    order, stats dict). FP32 ensembles certify at max(tol, 2e-6)."""
+    order, stats dict). FP32 ensembles certify at max(tol, 2e-6).
+    ``precond``: "auto" (the full-grid Fourier |q| preconditioner for both
+    kinds), "fourier" (13-tap stencil), "fourier-fft" or "none"."""
     return _ensemble_call(problems, False, tol, max_iter, use_pr, precision, precond,
                           check_every, use_graph, autotune, keep_best, stall_limit,
                           light_result)

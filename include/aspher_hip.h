@@ -308,8 +308,9 @@ typedef struct {
     int max_iter;            /* 5000 */
     int use_pr;              /* 1 */
     char precision[16];      /* "float" (FP32 + FP64 reductions) | "double" */
-    char precond[16];        /* "auto" (stencil for fft, Fourier for periodic),
-                                "none", "fourier", "fourier-fft" */
+    char precond[16];        /* "auto" (the full-grid Fourier |q| preconditioner
+                                for both kinds, as "fourier-fft"), "none",
+                                "fourier" (13-tap stencil), "fourier-fft" */
     int check_every;         /* 8 */
     int use_graph;           /* 1 */
     int autotune;            /* 1 */

@@ -45,7 +45,7 @@ def test_ensemble_fp64_matches_single_solves(hc):
         assert r.status == "converged", (b, Ns, r.status, r.status_reason)
         s = hc.ContactSolver(Ns, backend=b)
         ref = s.solve(pr["gap"], pr["p_nominal"], tol=1e-12, max_iter=20000,
-                      precond="fourier" if b == "fft" else "fourier-fft")
+                      precond="fourier-fft")
         assert ref.converged
         p, q = r.pressure.ravel(), ref.pressure.ravel()
         assert np.linalg.norm(p - q) / np.linalg.norm(q) < 1e-8
@@ -59,14 +59,17 @@ def test_ensemble_fp64_matches_single_solves(hc):
         assert r.stage_stats[0]["name"] == "ensemble:%d" % Ns
 
 
-def test_ensemble_graph_replays_equal_eager(hc):
+@pytest.mark.parametrize("precond", ["auto", "fourier"])
+def test_ensemble_graph_replays_equal_eager(hc, precond):
     """The captured iteration replayed must give bit-identical results to the
-    same sequence launched eagerly (graph replays with host polls between)."""
+    same sequence launched eagerly (graph replays with host polls between),
+    with the Fourier preconditioner's batched convolutions ("auto") and with
+    the stencil's tile work queue ("fourier")."""
     probs = _problems(hc, SPEC, seed0=10)
     ra, sa = hc.solve_ensemble(probs, tol=1e-9, precision="double", use_graph=True,
-                               check_every=3)
+                               check_every=3, precond=precond)
     rb, sb = hc.solve_ensemble(probs, tol=1e-9, precision="double", use_graph=False,
-                               check_every=5)
+                               check_every=5, precond=precond)
     assert sa["graph"] and not sb["graph"]
     for a, b in zip(ra, rb):
         assert a.status == b.status == "converged"
@@ -183,8 +186,7 @@ def test_ensemble_fp64_1024_group_matches_single_solves(hc):
         b, Ns = pr["backend"], pr["grid_size"]
         assert r.status == "converged", (b, r.status, r.status_reason)
         ref = hc.ContactSolver(Ns, backend=b).solve(
-            pr["gap"], pr["p_nominal"], tol=tol, max_iter=20000,
-            precond="fourier" if b == "fft" else "fourier-fft")
+            pr["gap"], pr["p_nominal"], tol=tol, max_iter=20000, precond="fourier-fft")
         assert ref.converged
         p, q = r.pressure.ravel(), ref.pressure.ravel()
         rel = np.linalg.norm(p - q) / np.linalg.norm(q)
