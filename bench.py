@@ -394,8 +394,11 @@ def main():
     t_fe_so, r_fe = timed(lambda: run_solve(hc, Ns, g.ptr, "float_then_double", True, ff,
                                             precond_engine="fft"), args.steps, 1)
     log("[bench]   %.3f s  %s" % (t_fe_so, r_fe))
+    t_fe64_so, r_fe64 = timed(lambda: run_solve(hc, Ns, g.ptr, "double", True, ff,
+                                                precond_engine="fft"), args.steps, 1)
+    log("[bench]   fp64: %.3f s  %s" % (t_fe64_so, r_fe64))
     hc.clear_operator_cache()
-    for r in (r_mixed, r_fp64, r_act, r_fe):
+    for r in (r_mixed, r_fp64, r_act, r_fe, r_fe64):
         assert r.status == "converged", r
 
     pm, pd = r_mixed.pressure.ravel(), r_fp64.pressure.ravel()
@@ -530,6 +533,8 @@ def main():
         "accuracy": accuracy,
         "fourier_engine": dict(
             solve_only_s=round(t_fe_so, 4), iterations=r_fe.iterations,
+            fp64_solve_only_s=round(t_fe64_so, 4), fp64_iterations=r_fe64.iterations,
+            speedup_mixed_vs_fp64=round(t_fe64_so / t_fe_so, 3),
             contact_area=r_fe.contact_area,
             pressure_rel_l2_vs_fp64=float(np.linalg.norm(r_fe.pressure.ravel() - pd)
                                           / np.linalg.norm(pd)),

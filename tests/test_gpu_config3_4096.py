@@ -1,7 +1,10 @@
 """The headline workload at its real size: free-space rough contact at
 Ns = 4096 (H = 0.8, rms 0.002, |k| in [4, 64], p_nominal 0.005, tol 1e-6,
-nested from 64), mixed precision (float_then_double), FP64, and mixed with
-the active set. Each returned pressure is certified by the oracle's
+nested from 64), mixed precision (float_then_double), FP64, mixed with the
+active set, and mixed with the full-grid Fourier preconditioner on every
+level (precond_engine="fft", src/nested_solve.cpp:711-723, where the
+automatic choice takes the stencil at this contact fraction). Each returned
+pressure is certified by the oracle's
 independent FP64 operator (scipy.fft of the C Love table), and the mixed
 result must match the FP64 one: contact area within 1e-4, pressure and mean
 gap close."""
@@ -27,13 +30,15 @@ def c3():
         "mixed": hc.solve_nested(precision="float_then_double", **kw),
         "fp64": hc.solve_nested(precision="double", **kw),
         "active": hc.solve_nested(precision="float_then_double", active_set=True, **kw),
+        "mixed_fft": hc.solve_nested(precision="float_then_double", precond_engine="fft",
+                                     **kw),
     }
     hc.clear_operator_cache()
     op = O.FreeSpaceOperator(NS)
     return g, out, op
 
 
-@pytest.mark.parametrize("which", ["mixed", "fp64", "active"])
+@pytest.mark.parametrize("which", ["mixed", "fp64", "active", "mixed_fft"])
 def test_c3_certified_by_oracle(c3, which):
     g, out, op = c3
     r = out[which]
@@ -50,7 +55,7 @@ def test_c3_certified_by_oracle(c3, which):
 def test_c3_mixed_matches_fp64(c3):
     _, out, _ = c3
     m, d, a = out["mixed"], out["fp64"], out["active"]
-    for r in (m, a):
+    for r in (m, a, out["mixed_fft"]):
         assert abs(r.contact_area - d.contact_area) <= 1e-4
         p, q = r.pressure.ravel(), d.pressure.ravel()
         assert np.linalg.norm(p - q) / np.linalg.norm(q) <= 1e-4
