@@ -184,7 +184,7 @@ __device__ __forceinline__ void load16(const double* __restrict__ p, double* v) 
 }
 
 // ── columns: forward . symbol . inverse (transposed in, row-major out) ─────
-template <typename T, int SYM>
+template <typename T, int SYM, int LOGN>
 __device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restrict__ ktab,
                                        int kp, double sc) {
     if constexpr (SYM == 1) {
@@ -193,7 +193,12 @@ __device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restric
         return m2 > 0.0 ? static_cast<T>(sc / sqrt(m2)) : T(0);
     } else {
         const int kof = (ky < (n >> 1)) ? ky : ky - n;
-        const float h = (float)sqrt((double)kx * kx + (double)kof * kof);
+        // |k| rounded once to float. Up to n = 4096, kx^2 + kof^2 <= n^2/2 is
+        // an exact float and the correctly rounded FP32 root of it is that
+        // rounding (checked for every integer below 2^24); beyond, FP64.
+        float h;
+        if constexpr (LOGN <= 12) h = sqrtf((float)(kx * kx + kof * kof));
+        else h = (float)sqrt((double)kx * kx + (double)kof * kof);
         const float w = (kx == 0 && ky == 0) ? 0.0f : (float)sc * h;
         return static_cast<T>(w);
     }
@@ -320,7 +325,7 @@ k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
 #pragma unroll
                 for (int v = 0; v < Nl; ++v) {
                     const int ky = Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v));
-                    const T s = live ? symbol_rb<T, SYM>(kx, ky, n, ktab, kp, sc) : T(0);
+                    const T s = live ? symbol_rb<T, SYM, LOGN>(kx, ky, n, ktab, kp, sc) : T(0);
                     r[q * Nl + v] = scale(r[q * Nl + v], s);
                 }
         }

@@ -296,6 +296,8 @@ def main():
                     help="only the matvec timing (for profiler runs)")
     ap.add_argument("--solve-once", choices=("mixed", "fp64"),
                     help="one cached-operator solve of the given precision (for profiler runs)")
+    ap.add_argument("--precond-engine", choices=("auto", "stencil", "fft"), default="auto",
+                    help="preconditioner engine of --solve-once")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every kernel of --solve-once / --ensemble-only (no hipGraph "
                          "replay): rocprofv3 kernel tracing faults on replayed graphs")
@@ -328,9 +330,11 @@ def main():
         g = DeviceArray.from_numpy(gap, np.float64)
         prec = "float_then_double" if args.solve_once == "mixed" else "double"
         gr = not args.no_graph
-        run_solve(hc, Ns, g.ptr, prec, True, args.float_floor, graph=gr)   # builds the operators
+        pe = args.precond_engine
+        run_solve(hc, Ns, g.ptr, prec, True, args.float_floor, graph=gr,
+                  precond_engine=pe)   # builds the operators
         t0 = time.perf_counter()
-        r = run_solve(hc, Ns, g.ptr, prec, True, args.float_floor, graph=gr)
+        r = run_solve(hc, Ns, g.ptr, prec, True, args.float_floor, graph=gr, precond_engine=pe)
         log("[bench] %s solve %.3f s %s" % (args.solve_once, time.perf_counter() - t0, r))
         return
 
