@@ -459,7 +459,26 @@ SpectralPlan<T>::SpectralPlan(int n_, int rin, int cin, int rout, int cout,
         int Bf = tune.row_batch > 0 ? B : std::max(1, 1024 / (Lr / 16));
         while (tune.row_batch <= 0 && Bf > B && Bf * rowlds > 147456) --Bf;
         row_batch_fwd_ = std::max(Bf, B);
-        pitchT_ = (std::max(rows_in, rows_out) + 31) / 32 * 32 + 32;
+        // small transforms: the batches above would put a whole pass in one
+        // or two workgroups (64 rows of a 128-point transform in one
+        // 1024-thread group), so they are halved until a pass has 256
+        // workgroups (one per CU) or a single row/column each. Measured on
+        // the nested 4096^2 solves (levels 64^2 .. 1024^2): mixed with the
+        // Fourier engine 0.0904 -> 0.0875 s, with the stencil 0.3455 ->
+        // 0.3436 s; targets of 16 and 64 groups gained less.
+        {
+            const int nr = std::max(rows_in, rows_out);
+            auto spread = [](int v, int count) {
+                while (v > 1 && (count + v - 1) / v < 256) v = (v + 1) / 2;
+                return v;
+            };
+            if (tune.row_batch <= 0) {
+                row_batch_ = spread(row_batch_, nr);
+                row_batch_fwd_ = std::max(spread(row_batch_fwd_, nr), row_batch_);
+            }
+            if (tune.col_width <= 0) col_width_ = spread(col_width_, nh);
+        }
+        pitchT_ =(std::max(rows_in, rows_out) + 31) / 32 * 32 + 32;
         // The column pass writes its result back into AT, transposed: every
         // store is a contiguous run, where row-major stores of single
         // elements into BR were written back ~2.15x over (measured at
