@@ -246,6 +246,42 @@ def test_stencil_small_and_ragged_grids(hc, Ns, radius):
     assert np.linalg.norm(zf - rf) / np.linalg.norm(rf) < 1e-5
 
 
+def _disk_mask(Ns, rng, n_disks, rmax):
+    """Contact patches: n_disks periodic disks (one across the corner), so most
+    64x16 tiles of the grid hold no contact point."""
+    iy, ix = np.meshgrid(np.arange(Ns), np.arange(Ns), indexing="ij")
+    m = np.zeros((Ns, Ns), dtype=bool)
+    centres = [(0, 0)] + [tuple(rng.integers(0, Ns, 2)) for _ in range(n_disks - 1)]
+    for cy, cx in centres:
+        r = rng.uniform(1.0, rmax)
+        dy = (iy - cy + Ns // 2) % Ns - Ns // 2
+        dx = (ix - cx + Ns // 2) % Ns - Ns // 2
+        m |= dy * dy + dx * dx <= r * r
+    return m.ravel()
+
+
+@pytest.mark.parametrize("Ns,radius,n_disks", [(512, 2, 6), (200, 2, 3), (1024, 3, 12),
+                                               (300, 7, 4)])
+def test_stencil_clustered_masks_skip_empty_tiles(hc, Ns, radius, n_disks):
+    """Masks of a few contact patches (ragged grids, a patch across the
+    periodic corner, compile-time and run-time radius): tiles without a
+    contact point are skipped by the stencil, so the result must still match
+    the oracle and be exactly zero off the mask."""
+    rng = np.random.default_rng(7 * Ns + radius)
+    mask = _disk_mask(Ns, rng, n_disks, max(4.0, Ns / 24))
+    tiles = mask.reshape(Ns, Ns)[: Ns // 16 * 16, : Ns // 64 * 64]
+    empty = ~tiles.reshape(Ns // 16, 16, Ns // 64, 64).any(axis=(1, 3))
+    assert empty.mean() > 0.5
+    g = rng.uniform(-1, 1, Ns * Ns)
+    ref = O.StencilPreconditioner(Ns, radius)
+    r = ref.apply(g, mask)
+    for precision, gate in ((64, 1e-12), (32, 1e-5)):
+        z = _precond("fourier", Ns, g, mask, precision=precision, radius=radius)
+        rr = r if precision == 64 else ref.apply(g.astype(np.float32), mask).astype(np.float64)
+        assert np.linalg.norm(z - rr) / np.linalg.norm(rr) < gate, precision
+        assert np.all(z[~mask] == 0.0), precision
+
+
 def test_preconditioners_offset_invariant_symmetric(hc):
     """tests/test_precond.cpp:27-66."""
     Ns = 32
