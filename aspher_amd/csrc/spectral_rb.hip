@@ -98,19 +98,26 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
     };
 
     cx<T> r[16];
-    // plain input on an even pitch and width from a pair-aligned base: both
-    // reals of a packed element in one access
-    const bool pairs =
-        IN == 0 && ((x_pitch | ncols) & 1) == 0 && pair_aligned(x, sizeof(T));
+    // plain or masked input on an even pitch and width from a pair-aligned
+    // base: both reals of a packed element in one access (and their two
+    // mask bytes in one)
+    const bool pairs = (IN == 0 || IN == 1) && ((x_pitch | ncols) & 1) == 0 &&
+                       pair_aligned(x, sizeof(T)) &&
+                       (IN == 0 || (reinterpret_cast<std::uintptr_t>(mask) & 1u) == 0);
     if (pairs) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int p = Sh::pos(0, th, v);
             r[v] = {T(0), T(0)};
             if ((Z && v >= 8) || row >= nrows || 2 * p >= ncols) continue;
-            const Pair<T> q =
-                *reinterpret_cast<const Pair<T>*>(x + (size_t)row * x_pitch + 2 * p);
-            r[v] = {q.a, q.b};
+            const std::size_t i = (size_t)row * x_pitch + 2 * p;
+            const Pair<T> q = *reinterpret_cast<const Pair<T>*>(x + i);
+            if constexpr (IN == 1) {
+                const unsigned mm = *reinterpret_cast<const unsigned short*>(mask + i);
+                r[v] = {(mm & 0xffu) ? T(q.a - gmean) : T(0), (mm >> 8) ? T(q.b - gmean) : T(0)};
+            } else {
+                r[v] = {q.a, q.b};
+            }
         }
     } else {
 #pragma unroll
@@ -448,16 +455,26 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
     }
     rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
     double acc = 0.0;
-    const bool pairs =
-        OUT == 0 && ((y_pitch | ncols) & 1) == 0 && pair_aligned(y, sizeof(T));
+    const bool pairs = (OUT == 0 || OUT == 1) && ((y_pitch | ncols) & 1) == 0 &&
+                       pair_aligned(y, sizeof(T)) &&
+                       (OUT == 0 || (reinterpret_cast<std::uintptr_t>(mask) & 1u) == 0);
     if (row < nrows && pairs) {
-        // plain output on an even pitch and width: pair stores
+        // plain or masked output on an even pitch and width: pair stores
+        // (and the two mask bytes in one load)
 #pragma unroll
         for (int v = 0; v < (H ? 8 : 16); ++v) {
             const int p = Sh::pos(0, th, v);
             if (2 * p >= ncols) continue;
-            *reinterpret_cast<Pair<T>*>(y + (size_t)row * y_pitch + 2 * p) =
-                Pair<T>{r[v].re, r[v].im};
+            const std::size_t i = (size_t)row * y_pitch + 2 * p;
+            T a = r[v].re, b = r[v].im;
+            if constexpr (OUT == 1) {
+                const unsigned mm = *reinterpret_cast<const unsigned short*>(mask + i);
+                if (mm & 0xffu) acc += (double)a;
+                else a = T(0);
+                if (mm >> 8) acc += (double)b;
+                else b = T(0);
+            }
+            *reinterpret_cast<Pair<T>*>(y + i) = Pair<T>{a, b};
         }
     } else if (row < nrows) {
 #pragma unroll
