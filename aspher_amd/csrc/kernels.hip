@@ -320,8 +320,8 @@ k_stencil(int Ns, const T* __restrict__ g, const std::uint8_t* __restrict__ mask
         }
     };
     // Tiles without a contact point have no output on the mask and add
-    // nothing to zsum; at a few percent contact most tiles are such (about
-    // seven in eight at 7% contact on a 4096^2 rough surface), so each tile
+    // nothing to zsum; at a few percent contact most tiles are such (83% of
+    // them at 6.9% contact on a 4096^2 rough surface), so each tile
     // is first tested on its 64 x 16 mask bytes, and only tiles with a
     // contact point fetch their halo. Each thread tests 4 consecutive bytes
     // of one tile row: one 4-byte load where Ns is a multiple of 4 (a
