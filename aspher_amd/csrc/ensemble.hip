@@ -483,7 +483,6 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
         finalize<Real>(part.p, 0, 3, o1, d1, scal.p, STEP_EVAL1, s, ev);
         V::eval2(Ntot, P, G, bmask.p, scal.p, part.p, 0, s, ev);
         finalize<Real>(part.p, 0, 5, kS5, d2, scal.p, STEP_B_EVAL2, s, ev);
-        if (opt.keep_best) V::copy_if(Ntot, P, bbest.p, scal.p, S_SAVEBEST, s, ev);
         if (has_pc) precondition();
         V::cgdot(Ntot, G, GP, Z, bmask.p, scal.p, has_pc, part.p, 0, s, ev);
         finalize<Real>(part.p, 0, 3, kS3, dg, scal.p, STEP_CGDOT, s, ev);
@@ -493,7 +492,9 @@ void ensemble_impl(const std::vector<EnsembleInstance>& inst, const EnsembleOpti
         finalize<Real>(part.p, 0, 2, kS2, dl1, scal.p, STEP_LINE1, s, ev);
         V::line2(Ntot, R, T_, bmask.p, scal.p, part.p, 0, s, ev);
         finalize<Real>(part.p, 0, 1, kS1, dl2, scal.p, STEP_LINE2, s, ev);
-        V::update(Ntot, P, P, T_, G, scal.p, part.p, 0, s, ev);
+        // the best-iterate copy rides on the update (P is unchanged until then)
+        V::update(Ntot, P, P, T_, G, scal.p, part.p, 0, s, ev,
+                  opt.keep_best ? bbest.p : nullptr, S_SAVEBEST);
         finalize<Real>(part.p, 0, 2, ou, du, scal.p, STEP_UPDATE, s, ev);
         V::rescale(Ntot, P, scal.p, s, ev);
     };

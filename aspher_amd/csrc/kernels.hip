@@ -586,23 +586,29 @@ k_line2(std::int64_t N, const T* __restrict__ r, const T* __restrict__ t,
 
 // ── update, overlap step, load rescale (src/contact_solver.cpp:477-501) ────
 
+// best != nullptr: where the instance's scalar best_flag is set, the
+// incoming pressure is also stored to best (the best iterate, marked by the
+// evaluation; the pressure is not written between that and this update).
 template <typename T, int VW>
 __global__ void __launch_bounds__(256)
 k_update(std::int64_t N, const T* pin, T* pout, const T* __restrict__ t,
          const T* __restrict__ g, const double* __restrict__ scal,
-         double* __restrict__ part, Ens ens) {
+         double* __restrict__ part, Ens ens, T* __restrict__ best, int best_flag) {
     __shared__ double red[8];
     ENS_SETUP_V(N, VW);
     pin += o_; pout += o_; t += o_; g += o_;
     scal = ENS_SCAL(scal);
     double* pr = ENS_PART(part);
+    const bool save = best != nullptr && scal[best_flag] != 0.0;
+    if (save) best += o_;
     if (scal[S_OK] == 0.0) {
         // no usable curvature: the pressure is unchanged
-        if (pout != pin) {
+        if (pout != pin || save) {
             ENS_LOOP(i) {
                 T pa[VW];
                 ldv<VW>(pin + i, pa);
-                stv<VW>(pout + i, pa);
+                if (pout != pin) stv<VW>(pout + i, pa);
+                if (save) stv<VW>(best + i, pa);
             }
         }
         if (threadIdx.x == 0) {
@@ -618,6 +624,7 @@ k_update(std::int64_t N, const T* pin, T* pout, const T* __restrict__ t,
         T pa[VW], ga[VW], ta[VW];
         ldv<VW>(pin + i, pa);
         ldv<VW>(g + i, ga);
+        if (save) stv<VW>(best + i, pa);
         bool any = false;
 #pragma unroll
         for (int k = 0; k < VW; ++k) any = any || pa[k] > T(0);
@@ -664,22 +671,6 @@ k_rescale(std::int64_t N, T* __restrict__ p, const double* __restrict__ scal, En
             for (int k = 0; k < VW; ++k) pa[k] = T(pa[k] * f);
         }
         stv<VW>(p + i, pa);
-    }
-}
-
-// best iterate: y = x where the instance's scalar `flag` is nonzero
-template <typename T, int VW>
-__global__ void __launch_bounds__(256)
-k_copy_if(std::int64_t N, const T* __restrict__ x, T* __restrict__ y,
-          const double* __restrict__ scal, int flag, Ens ens) {
-    ENS_SETUP_V(N, VW);
-    x += o_; y += o_;
-    scal = ENS_SCAL(scal);
-    if (scal[flag] == 0.0) return;
-    ENS_LOOP(i) {
-        T a[VW];
-        ldv<VW>(x + i, a);
-        stv<VW>(y + i, a);
     }
 }
 
@@ -1533,21 +1524,18 @@ void VecOps<T>::line2(std::int64_t N, const T* r, const T* t, const std::uint8_t
 template <typename T>
 void VecOps<T>::update(std::int64_t N, const T* pin, T* pout, const T* t, const T* g,
                        const double* scal, double* part, int nblk, hipStream_t s,
-                       const Ens& ens) {
-    if (vec4_ok(N, ens, pin, pout, t, g)) { LE((k_update<T, 4>), N, pin, pout, t, g, scal, part, ens); }
-    else { LE((k_update<T, 1>), N, pin, pout, t, g, scal, part, ens); }
+                       const Ens& ens, T* best, int best_flag) {
+    if (vec4_ok(N, ens, pin, pout, t, g, best)) {
+        LE((k_update<T, 4>), N, pin, pout, t, g, scal, part, ens, best, best_flag);
+    } else {
+        LE((k_update<T, 1>), N, pin, pout, t, g, scal, part, ens, best, best_flag);
+    }
 }
 template <typename T>
 void VecOps<T>::rescale(std::int64_t N, T* p, const double* scal, hipStream_t s,
                         const Ens& ens) {
     if (vec4_ok(N, ens, p)) { LES((k_rescale<T, 4>), N, p, scal, ens); }
     else { LES((k_rescale<T, 1>), N, p, scal, ens); }
-}
-template <typename T>
-void VecOps<T>::copy_if(std::int64_t N, const T* x, T* y, const double* scal, int flag,
-                        hipStream_t s, const Ens& ens) {
-    if (vec4_ok(N, ens, x, y)) { LES((k_copy_if<T, 4>), N, x, y, scal, flag, ens); }
-    else { LES((k_copy_if<T, 1>), N, x, y, scal, flag, ens); }
 }
 template <typename T>
 void VecOps<T>::objective(std::int64_t N, const T* p, const T* u, const T* g0s,

@@ -80,15 +80,14 @@ template <typename T> struct VecOps {
                       const Ens& ens = Ens{});
     // projected update + overlap step pout = update(pin) (predicated on S_OK,
     // else pout = pin; pin == pout allowed): partials {changed, total}
+    // best != nullptr: also stores pin to best where scal[best_flag] is set
+    // (the best-iterate copy, fused)
     static void update(std::int64_t N, const T* pin, T* pout, const T* t, const T* g,
                        const double* scal, double* part, int nblk, hipStream_t s,
-                       const Ens& ens = Ens{});
+                       const Ens& ens = Ens{}, T* best = nullptr, int best_flag = 0);
     // load rescale (predicated on S_OK)
     static void rescale(std::int64_t N, T* p, const double* scal, hipStream_t s,
                         const Ens& ens = Ens{});
-    // y = x where scal[flag] != 0 (per instance)
-    static void copy_if(std::int64_t N, const T* x, T* y, const double* scal, int flag,
-                        hipStream_t s, const Ens& ens = Ens{});
     // {sum p(0.5u+g0s), sum p^2, sum p u}
     static void objective(std::int64_t N, const T* p, const T* u, const T* g0s,
                           double* part, int nblk, hipStream_t s);

@@ -392,7 +392,6 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             finalize<Real>(part, nblk, 3, o1, d1, scal, STEP_EVAL1, s, eg);
             V::eval2(N, P, G, mask, scal, part, nblk, s, eg);
             finalize<Real>(part, nblk, 5, kS5, d2, scal, STEP_S_EVAL2, s, eg);
-            if (PB) V::copy_if(N, P, PB, scal, S_SAVEBEST, s, eg);
             if (has_pc) {
                 if (timed) tm_pc.start(s);
                 pc->apply<Real>(G, mask, Z, scal, part, s, S_ACTIVE, false);
@@ -408,7 +407,9 @@ void solve_impl(Context& ctx, Operator& op, const Real* g0, Real p_bar,
             finalize<Real>(part, nblk, 2, kS2, dl1, scal, STEP_LINE1, s, eg);
             V::line2(N, R, T_, mask, scal, part, nblk, s, eg);
             finalize<Real>(part, nblk, 1, kS1, dl2, scal, STEP_LINE2, s, eg);
-            V::update(N, P, P, T_, G, scal, part, nblk, s, eg);
+            // the best-iterate copy rides on the update (P is unchanged
+            // until then)
+            V::update(N, P, P, T_, G, scal, part, nblk, s, eg, PB, S_SAVEBEST);
             finalize<Real>(part, nblk, 2, ou, du, scal, STEP_UPDATE, s, eg);
             V::rescale(N, P, scal, s, eg);
         };
