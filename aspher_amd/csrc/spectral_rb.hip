@@ -559,14 +559,15 @@ void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
 // Concurrent workgroups of a kernel on the device (occupancy x CUs), cached.
 int resident_blocks(const void* fn, int block, std::size_t lds) {
     static std::mutex mu;
-    static std::map<std::tuple<const void*, int, std::size_t>, int> done;
+    static std::map<std::tuple<int, const void*, int, std::size_t>, int> done;
+    int dev = 0;
+    ASPHER_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(mu);
-    const auto key = std::make_tuple(fn, block, lds);
+    const auto key = std::make_tuple(dev, fn, block, lds);   // per device: CU counts differ
     auto it = done.find(key);
     if (it != done.end()) return it->second;
-    int per_cu = 0, dev = 0, cus = 0;
+    int per_cu = 0, cus = 0;
     ASPHER_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds));
-    ASPHER_HIP(hipGetDevice(&dev));
     ASPHER_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int r = std::max(1, per_cu) * std::max(1, cus);
     done[key] = r;
