@@ -211,6 +211,21 @@ struct Shape {
 __device__ __forceinline__ int lds_ix(int p) { return p + (p >> 4); }
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 4) + 1; }
 
+// Element -> slot layouts of LDS buffers. The LDS serves a wave's 4- and
+// 8-byte accesses in two halves of 32 lanes; the slots one half touches are
+// conflict-free when they differ mod 32 (measured: a 2-way conflict costs
+// two cycles per access, a 32-way one 60).
+// Skew16 (one slot of skew per 16 elements) keeps stride-16 and
+// stride-32-pair patterns apart but folds the two ends of a 32-element run
+// onto one bank; Skew32x2 (two slots per 32) keeps runs of 32 and the
+// stride-32 pairs apart. Both fit in lds_len(n) slots.
+struct Skew16 {
+    static __host__ __device__ constexpr int ix(int p) { return p + (p >> 4); }
+};
+struct Skew32x2 {
+    static __host__ __device__ constexpr int ix(int p) { return p + 2 * (p >> 5); }
+};
+
 // twiddles W^{v*idx}, v = 0..N-1, applied to a group from one FP64 table
 // read w = tw[idx] (tw[t] = exp(-2 pi i t/len)).
 // FP64 data: powers as a running FP64 product (at most 15 roundings,
@@ -360,10 +375,11 @@ struct NoHook {
 // base(th) + off(q, v) = Shape::pos(s, th*G + q, v): a radix-16 stage holds
 // one group (off = v*S), the last stage of a radix below 16 holds G groups
 // of stride 1 (base = 16 th, off = q*N + v). The skew does not carry
-// between the two parts (base mod 16 plus off mod 16 stays below 16 in both
-// cases), so lds_ix(base + off) = lds_ix(base) + off + off/16: one address
-// per thread and compile-time offsets.
-template <int LOGN, int s>
+// between the two parts (for Skew16, base mod 16 plus off mod 16 stays below
+// 16 in both cases; for Skew32x2, the same mod 32 on the exchanges it is
+// used for), so Lay::ix(base + off) = Lay::ix(base) + Lay::ix(off): one
+// address per thread and compile-time offsets.
+template <int LOGN, int s, typename Lay = Skew16>
 struct Grouping {
     using Sh = Shape<LOGN>;
     static constexpr int N = Sh::N(s), S = Sh::S(s), G = Sh::G(s);
@@ -374,9 +390,22 @@ struct Grouping {
     }
     static constexpr int off(int q, int v) {
         const int o = G == 1 ? v * S : q * N + v;
-        return o + (o >> 4);
+        return Lay::ix(o);
     }
 };
+
+// Layout of the exchange between stages e and e + 1 (both directions). For
+// FP32 data of transforms with whole waves per transform (n >= 1024) every
+// exchange but the last takes Skew32x2: its groupings are runs of 32 (or
+// two runs of 16) on one side and runs or stride-32 pairs on the other,
+// which Skew16 conflicts on and Skew32x2 does not; the last exchange's
+// stride-16 side needs Skew16. Both stay linear in the same way as above:
+// ix(base + off) = ix(base) + ix(off) for every base and offset used, so
+// addressing stays one address per thread plus compile-time offsets.
+template <typename T, int LOGN, int e>
+using ExLayout = std::conditional_t<std::is_same_v<T, float> && LOGN >= 10 &&
+                                        e + 2 < Shape<LOGN>::m,
+                                    Skew32x2, Skew16>;
 
 // Regroup registers from stage sa's grouping to stage sb's through LDS.
 // `hook` runs between the LDS writes and the barrier, where the registers
@@ -386,17 +415,18 @@ struct Grouping {
 template <typename T, int LOGN, int sa, int sb, typename Hook = NoHook>
 __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
                                          const Hook& hook = Hook{}) {
-    using A = Grouping<LOGN, sa>;
-    using B = Grouping<LOGN, sb>;
+    using Lay = ExLayout<T, LOGN, (sa < sb ? sa : sb)>;
+    using A = Grouping<LOGN, sa, Lay>;
+    using B = Grouping<LOGN, sb, Lay>;
     __syncthreads();
-    cx<T>* wa = lds + lds_ix(A::base(th));
+    cx<T>* wa = lds + Lay::ix(A::base(th));
 #pragma unroll
     for (int q = 0; q < A::G; ++q)
 #pragma unroll
         for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v];
     hook();
     __syncthreads();
-    const cx<T>* rbp = lds + lds_ix(B::base(th));
+    const cx<T>* rbp = lds + Lay::ix(B::base(th));
 #pragma unroll
     for (int q = 0; q < B::G; ++q)
 #pragma unroll
@@ -410,10 +440,11 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
 template <typename T, int LOGN, int sa, int sb, typename Hook = NoHook>
 __device__ __forceinline__ void exchange(cx<T>* r, T* lds, int th,
                                          const Hook& hook = Hook{}) {
-    using A = Grouping<LOGN, sa>;
-    using B = Grouping<LOGN, sb>;
-    T* wa = lds + lds_ix(A::base(th));
-    const T* rbp = lds + lds_ix(B::base(th));
+    using Lay = ExLayout<T, LOGN, (sa < sb ? sa : sb)>;
+    using A = Grouping<LOGN, sa, Lay>;
+    using B = Grouping<LOGN, sb, Lay>;
+    T* wa = lds + Lay::ix(A::base(th));
+    const T* rbp = lds + Lay::ix(B::base(th));
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < A::G; ++q)

@@ -34,6 +34,24 @@ __host__ __device__ __forceinline__ bool pair_aligned(const void* p, std::size_t
     return (reinterpret_cast<std::uintptr_t>(p) % (2 * elt)) == 0;
 }
 
+// Slot of natural-order element k in a row pass's LDS row (the
+// digit-reversed write and packed-spectrum split of the forward pass, the
+// transposed load and gather of the inverse). The B rows of a workgroup are
+// read and written together there, lane t on row t % B at element t / B + c:
+// FP32 rows take Skew32x2, and rows start 32 / B slots apart mod 32
+// (row_stride), so the lanes of a half-wave fall in distinct banks.
+template <typename T>
+__device__ __forceinline__ int row_ix(int k) {
+    if constexpr (std::is_same_v<T, float>) return rb::Skew32x2::ix(k);
+    else return rb::lds_ix(k);
+}
+template <typename T>
+__host__ __device__ inline int row_stride(int L, int B) {
+    const int base = rb::lds_len(L);
+    if (!std::is_same_v<T, float> || B < 2 || B > 32) return base;
+    return base + ((32 / B - base % 32) % 32 + 32) % 32;
+}
+
 __device__ __forceinline__ int xcd_block(int b, int G) {
     const int q = G >> 3, r = G & 7;
     const int x = b & 7;
@@ -53,7 +71,7 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
               RbBatch bt) {
     using Sh = rb::Shape<LOGL>;
     constexpr int L = Sh::n, n = 2 * L, TPR = Sh::threads;
-    constexpr int LL = rb::lds_len(L);
+    const int LL = row_stride<T>(L, B);
     extern __shared__ __align__(16) unsigned char smem[];
     {
         const int li = blockIdx.y, gi = bt.first + li;
@@ -136,7 +154,7 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
     for (int q = 0; q < Gl; ++q)
 #pragma unroll
         for (int v = 0; v < Nl; ++v)
-            lds[rb::lds_ix(Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v)))] = r[q * Nl + v];
+            lds[row_ix<T>(Sh::kidx(Sh::pos(Sh::m - 1, th * Gl + q, v)))] = r[q * Nl + v];
     __syncthreads();
     // store: rows of the workgroup fastest, so B consecutive rows of one kx
     // form one contiguous run of the transposed layout. Element e =
@@ -153,8 +171,8 @@ k_rb_rows_fwd(const T* __restrict__ x, int x_pitch, int ncols, int nrows,
         for (int i = 0; i < KI; ++i) {
             const int k = k0 + i * TPR;
             if (k > L) break;
-            const cx<T> Zk = lb[rb::lds_ix(k == L ? 0 : k)];
-            const cx<T> Zc = conjg(lb[rb::lds_ix(k == 0 ? 0 : L - k)]);
+            const cx<T> Zk = lb[row_ix<T>(k == L ? 0 : k)];
+            const cx<T> Zc = conjg(lb[row_ix<T>(k == 0 ? 0 : L - k)]);
             const cx<T> E = scale(Zk + Zc, T(0.5));
             const cx<T> D = Zk - Zc;
             const cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};
@@ -378,7 +396,7 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
         if (part) part += (std::size_t)gi * bt.pstride;
     }
     constexpr int L = Sh::n, TPR = Sh::threads;
-    constexpr int LL = rb::lds_len(L);
+    const int LL = row_stride<T>(L, B);
     constexpr int Gl = Sh::G(Sh::m - 1), Nl = Sh::N(Sh::m - 1);
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red[16];
@@ -410,7 +428,7 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const int k = k0 + j * TPR;
-            if (k <= L) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+            if (k <= L) lds_all[bb * LL + row_ix<T>(k)] = xv[j];
         }
     } else {
 #pragma unroll
@@ -424,7 +442,7 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
         for (int j = 0; j < J; ++j) {
             const int e = threadIdx.x + j * (int)blockDim.x;
             const int bb = e / (L + 1), k = e - bb * (L + 1);
-            if (e < lim) lds_all[bb * LL + rb::lds_ix(k)] = xv[j];
+            if (e < lim) lds_all[bb * LL + row_ix<T>(k)] = xv[j];
         }
     }
     __syncthreads();
@@ -439,7 +457,7 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
 #pragma unroll
         for (int v = 0; v < Nl; ++v) {
             const int k = kg + v * (L / Nl);
-            cx<T> Xk = lds[rb::lds_ix(k)], Xc = lds[rb::lds_ix(L - k)];
+            cx<T> Xk = lds[row_ix<T>(k)], Xc = lds[row_ix<T>(L - k)];
             if (k == 0) {
                 Xk.im = T(0);
                 Xc.im = T(0);
@@ -523,7 +541,7 @@ void launch_rows_fwd(const RbRowsFwd<T>& a, hipStream_t s) {
     if (!ok) throw std::invalid_argument("row transform: missing operand for its input mode");
     const int block = a.B * (L / 16);
     const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
-    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
+    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * row_stride<T>(L, a.B);
     const void* fn;
 #define RF_ARGS a.x, a.x_pitch, a.ncols, a.nrows, a.AT, a.pitchT, a.B, a.tw, a.mask, a.scal, \
         a.gmean_idx, a.table, a.Ns_embed, a.remap, a.bt
@@ -634,7 +652,7 @@ void launch_rows_inv(const RbRowsInv<T>& a, hipStream_t s) {
     if (!ok) throw std::invalid_argument("row transform: missing operand for its output mode");
     const int block = a.B * (L / 16);
     const dim3 grid((a.nrows + a.B - 1) / a.B, a.bt.count);
-    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * rb::lds_len(L);
+    const std::size_t lds = sizeof(cx<T>) * (std::size_t)a.B * row_stride<T>(L, a.B);
 #define RI_ARGS a.BR, a.pitch, a.nrows, a.B, a.tw, a.y, a.y_pitch, a.ncols, a.mask, a.part, a.remap, \
         a.bt, a.in_t
     if (a.mode == 0 && a.ncols <= L) {
