@@ -212,9 +212,10 @@ __device__ __forceinline__ int lds_ix(int p) { return p + (p >> 4); }
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 4) + 1; }
 
 // Element -> slot layouts of LDS buffers. The LDS serves a wave's 4- and
-// 8-byte accesses in two halves of 32 lanes; the slots one half touches are
-// conflict-free when they differ mod 32 (measured: a 2-way conflict costs
-// two cycles per access, a 32-way one 60).
+// 8-byte accesses in two halves of 32 lanes, whose slots are conflict-free
+// when they differ mod 32, and its 16-byte accesses in four quarters of 16
+// lanes, conflict-free when they differ mod 16 (measured: a 2-way conflict
+// costs two cycles per access in the first case, four in the second).
 // Skew16 (one slot of skew per 16 elements) keeps stride-16 and
 // stride-32-pair patterns apart but folds the two ends of a 32-element run
 // onto one bank; Skew32x2 (two slots per 32) keeps runs of 32 and the
@@ -395,17 +396,15 @@ struct Grouping {
 };
 
 // Layout of the exchange between stages e and e + 1 (both directions). For
-// FP32 data of transforms with whole waves per transform (n >= 1024) every
-// exchange but the last takes Skew32x2: its groupings are runs of 32 (or
-// two runs of 16) on one side and runs or stride-32 pairs on the other,
-// which Skew16 conflicts on and Skew32x2 does not; the last exchange's
-// stride-16 side needs Skew16. Both stay linear in the same way as above:
+// transforms with whole waves per transform (n >= 1024) every exchange but
+// the last takes Skew32x2: its groupings are runs of 32 (or two runs of 16)
+// on one side and runs or stride-32 pairs on the other, which Skew16
+// conflicts on and Skew32x2 does not, for 4-, 8- and 16-byte elements
+// alike; the last exchange's stride-16 side needs Skew16. Both stay linear in the same way as above:
 // ix(base + off) = ix(base) + ix(off) for every base and offset used, so
 // addressing stays one address per thread plus compile-time offsets.
 template <typename T, int LOGN, int e>
-using ExLayout = std::conditional_t<std::is_same_v<T, float> && LOGN >= 10 &&
-                                        e + 2 < Shape<LOGN>::m,
-                                    Skew32x2, Skew16>;
+using ExLayout = std::conditional_t<(LOGN >= 10 && e + 2 < Shape<LOGN>::m), Skew32x2, Skew16>;
 
 // Regroup registers from stage sa's grouping to stage sb's through LDS.
 // `hook` runs between the LDS writes and the barrier, where the registers

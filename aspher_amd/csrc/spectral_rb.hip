@@ -38,18 +38,19 @@ __host__ __device__ __forceinline__ bool pair_aligned(const void* p, std::size_t
 // digit-reversed write and packed-spectrum split of the forward pass, the
 // transposed load and gather of the inverse). The B rows of a workgroup are
 // read and written together there, lane t on row t % B at element t / B + c:
-// FP32 rows take Skew32x2, and rows start 32 / B slots apart mod 32
-// (row_stride), so the lanes of a half-wave fall in distinct banks.
+// rows take Skew32x2 and start P / B slots apart mod P (row_stride; P = 32
+// for 8-byte elements, 16 for 16-byte ones), so the lanes the LDS serves
+// together fall in distinct banks.
 template <typename T>
 __device__ __forceinline__ int row_ix(int k) {
-    if constexpr (std::is_same_v<T, float>) return rb::Skew32x2::ix(k);
-    else return rb::lds_ix(k);
+    return rb::Skew32x2::ix(k);
 }
 template <typename T>
 __host__ __device__ inline int row_stride(int L, int B) {
     const int base = rb::lds_len(L);
-    if (!std::is_same_v<T, float> || B < 2 || B > 32) return base;
-    return base + ((32 / B - base % 32) % 32 + 32) % 32;
+    constexpr int P = sizeof(cx<T>) >= 16 ? 16 : 32;
+    if (B < 2 || B > P) return base;
+    return base + ((P / B - base % P) % P + P) % P;
 }
 
 __device__ __forceinline__ int xcd_block(int b, int G) {
