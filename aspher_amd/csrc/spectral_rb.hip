@@ -231,18 +231,23 @@ __device__ __forceinline__ T symbol_rb(int kx, int ky, int n, const T* __restric
 }
 
 // Free-space column passes that exchange through a real (T) LDS buffer:
-// FP32, and FP64 at n = 8192, where the halved LDS lets a second 512-thread
-// workgroup be resident (measured 3.5% faster at 4096^2; at n = 4096 FP64
-// it was 15% slower and is not used).
+// FP64 at n = 8192, where the halved LDS lets a second 512-thread workgroup
+// be resident (measured 3.5% faster at 4096^2; at n = 4096 FP64 it was 15%
+// slower and is not used). FP32 columns exchange complex values: their
+// 8-byte exchanges fit two workgroups in LDS anyway, and with bank-conflict-
+// free layouts the halved barrier count wins (FP32 column pass at 4096^2:
+// 128 -> 122 us against the split exchanges).
 template <typename T, int LOGN, int SYM>
-constexpr bool kColsSplit = SYM == 0 && (sizeof(T) == 4 || LOGN >= 13);
+constexpr bool kColsSplit = SYM == 0 && sizeof(T) == 8 && LOGN >= 13;
 
-// Split columns: at most 128 VGPRs (4 waves per SIMD, two resident 512-thread
-// workgroups). Measured at 4096^2, FP32: the split exchanges are ~4% faster
-// than complex ones at this occupancy; forcing 80 VGPRs for a third
-// workgroup spilled and was ~12% slower.
+// Free-space column passes held to 128 VGPRs (4 waves per SIMD, two resident
+// 512-thread workgroups), FP32 and FP64 at n = 8192; forcing 80 VGPRs for a
+// third workgroup spilled and was ~12% slower.
 template <typename T, int LOGN, int SYM>
-__global__ void __launch_bounds__(512, (kColsSplit<T, LOGN, SYM> ? 4 : 1))
+constexpr bool kColsBound4 = SYM == 0 && (sizeof(T) == 4 || LOGN >= 13);
+
+template <typename T, int LOGN, int SYM>
+__global__ void __launch_bounds__(512, (kColsBound4<T, LOGN, SYM> ? 4 : 1))
 k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
           int ncols, int rows_in, int rows_out, int W, const cx<double>* __restrict__ tw,
           const T* __restrict__ ktab, int kp, double sc, T* __restrict__ kout,
@@ -274,8 +279,8 @@ k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
     // are adjacent within each store instruction (W-fold longer runs of
     // the strided row-major stores; -14% at n = 4096, FP32)
     const int c = tid % W, th = tid / W;
-    // table symbol: the exchanges pass real and imaginary parts separately
-    // through half the LDS (more resident workgroups)
+    // FP64 table symbol at n = 8192: the exchanges pass real and imaginary
+    // parts separately through half the LDS (more resident workgroups)
     using LT = std::conditional_t<kColsSplit<T, LOGN, SYM>, T, cx<T>>;
     LT* lds = reinterpret_cast<LT*>(smem) + c * LL;
     const int kx = tile * W + c;
@@ -300,10 +305,10 @@ k_rb_cols(const cx<T>* AT, int pitchT, cx<T>* BR, int pitch,
     // reads it reversed (kmap); without a map, one chunk per thread
     int km = th << 1;
     // FP32 table symbol: the inverse reuses the forward's stage-KS twiddle
-    // powers (15 values more per thread, within the split columns' 128
+    // powers (15 values more per thread, within the bounded columns' 128
     // VGPRs; the computed-symbol passes have no such bound and would lose a
     // resident workgroup)
-    constexpr int KS = kColsSplit<T, LOGN, SYM> && std::is_same_v<T, float>
+    constexpr int KS = kColsBound4<T, LOGN, SYM> && std::is_same_v<T, float>
                            ? rb::keep_stage<LOGN>() : -1;
     rb::TwKeep keep;
     if constexpr (SYM == 0) {
@@ -609,8 +614,7 @@ void launch_cols(const RbCols<T>& a, hipStream_t s) {
     constexpr int n = 1 << LOGN;
     const int block = a.W * (n / 16);
     const int tiles = (a.ncols + a.W - 1) / a.W;
-    const std::size_t elt = (a.sym == 0 && (sizeof(T) == 4 || LOGN >= 13)) ? sizeof(T)
-                                                                          : sizeof(cx<T>);
+    const std::size_t elt = (a.sym == 0 && kColsSplit<T, LOGN, 0>) ? sizeof(T) : sizeof(cx<T>);
     const std::size_t lds = elt * (std::size_t)a.W * rb::lds_len(n);
     int gx = tiles;
     if (a.sym != 3 && a.bt.count == 1) {
