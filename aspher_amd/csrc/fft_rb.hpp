@@ -406,6 +406,57 @@ struct Grouping {
 template <typename T, int LOGN, int e>
 using ExLayout = std::conditional_t<(LOGN >= 10 && e + 2 < Shape<LOGN>::m), Skew32x2, Skew16>;
 
+// Whether the exchange between groupings sa and sb stays inside groups of
+// `lanes` consecutive threads: each group's positions are the same set in
+// both groupings, so no element crosses to another group. `lanes` is the
+// fewest threads of one transform a wave can hold: 64 / W with W up to
+// 512 / threads transforms per workgroup (a column workgroup interleaves
+// its W columns across threads), 64 when one transform fills the wave.
+template <int LOGN, int sa, int sb>
+constexpr bool wave_local() {
+    using Sh = Shape<LOGN>;
+    constexpr int T = Sh::threads;
+    constexpr int Wmax = T >= 512 ? 1 : 512 / T;
+    constexpr int lanes = 64 / Wmax;
+    if (T < 64 || lanes < 1) return false;
+    auto pos = [](int s, int g, int v) {
+        return (g / Sh::S(s)) * Sh::N(s) * Sh::S(s) + v * Sh::S(s) + (g % Sh::S(s));
+    };
+    for (int g0 = 0; g0 < T; g0 += lanes) {
+        int lo[2] = {1 << 30, 1 << 30}, hi[2] = {-1, -1};
+        const int st[2] = {sa, sb};
+        for (int k = 0; k < 2; ++k)
+            for (int th = g0; th < g0 + lanes; ++th)
+                for (int q = 0; q < Sh::G(st[k]); ++q)
+                    for (int v = 0; v < Sh::N(st[k]); ++v) {
+                        const int p = pos(st[k], th * Sh::G(st[k]) + q, v);
+                        lo[k] = p < lo[k] ? p : lo[k];
+                        hi[k] = p > hi[k] ? p : hi[k];
+                    }
+        // 16 distinct positions per thread: equal bounds spanning exactly
+        // 16 * lanes positions mean equal sets
+        if (lo[0] != lo[1] || hi[0] != hi[1] || hi[0] - lo[0] + 1 != 16 * lanes) return false;
+    }
+    return true;
+}
+
+// Order a wave's LDS writes before its later LDS reads: the LDS executes one
+// wave's accesses in issue order, so this only keeps the compiler from
+// moving the reads above the writes (no instruction is emitted).
+__device__ __forceinline__ void wave_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exchanges of FP32 data that stay inside every wave are ordered by
+// wave_order instead of workgroup barriers.
+template <typename T, int LOGN, int sa, int sb>
+__device__ __forceinline__ void exchange_sync() {
+    if constexpr (std::is_same_v<T, float> && wave_local<LOGN, sa, sb>()) wave_order();
+    else __syncthreads();
+}
+
 // Regroup registers from stage sa's grouping to stage sb's through LDS.
 // `hook` runs between the LDS writes and the barrier, where the registers
 // of the data are free: a caller issues global loads there (the kernel
@@ -417,14 +468,14 @@ __device__ __forceinline__ void exchange(cx<T>* r, cx<T>* lds, int th,
     using Lay = ExLayout<T, LOGN, (sa < sb ? sa : sb)>;
     using A = Grouping<LOGN, sa, Lay>;
     using B = Grouping<LOGN, sb, Lay>;
-    __syncthreads();
+    exchange_sync<T, LOGN, sa, sb>();
     cx<T>* wa = lds + Lay::ix(A::base(th));
 #pragma unroll
     for (int q = 0; q < A::G; ++q)
 #pragma unroll
         for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v];
     hook();
-    __syncthreads();
+    exchange_sync<T, LOGN, sa, sb>();
     const cx<T>* rbp = lds + Lay::ix(B::base(th));
 #pragma unroll
     for (int q = 0; q < B::G; ++q)

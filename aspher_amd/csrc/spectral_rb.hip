@@ -477,6 +477,9 @@ k_rb_rows_inv(const cx<T>* __restrict__ BR, int pitch, int nrows, int B,
             r[q * Nl + v] = {S.re - D.im, S.im + D.re};
         }
     }
+    // every wave has gathered from the whole row before the transform's
+    // first exchange writes (it may be ordered within each wave only)
+    __syncthreads();
     rb::inverse<T, LOGL, H>(r, lds, th, tw, 2);
     double acc = 0.0;
     const bool pairs = (OUT == 0 || OUT == 1) && ((y_pitch | ncols) & 1) == 0 &&
@@ -751,6 +754,10 @@ k_rb_cols2(const cx<T>* __restrict__ AT, int pitchT, std::int64_t at_stride,
                 r[i] = {a * r[i].re + b * o[i].re, a * r[i].im + b * o[i].im};
             }
     }
+    // the other component has read this one's spectrum before the inverse's
+    // first exchange overwrites it (that exchange may be ordered within
+    // each wave only)
+    __syncthreads();
     rb::inverse<T, LOGN, ZP>(r, lds, th, tw, 1);
     if (live) {
         cx<T>* out = BR + c * br_stride;
