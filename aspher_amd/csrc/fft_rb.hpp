@@ -453,7 +453,7 @@ __device__ __forceinline__ void wave_order() {
 // wave_order instead of workgroup barriers.
 template <typename T, int LOGN, int sa, int sb>
 __device__ __forceinline__ void exchange_sync() {
-    if constexpr (std::is_same_v<T, float> && wave_local<LOGN, sa, sb>()) wave_order();
+    if constexpr (wave_local<LOGN, sa, sb>()) wave_order();
     else __syncthreads();
 }
 
@@ -495,23 +495,23 @@ __device__ __forceinline__ void exchange(cx<T>* r, T* lds, int th,
     using B = Grouping<LOGN, sb, Lay>;
     T* wa = lds + Lay::ix(A::base(th));
     const T* rbp = lds + Lay::ix(B::base(th));
-    __syncthreads();
+    exchange_sync<T, LOGN, sa, sb>();
 #pragma unroll
     for (int q = 0; q < A::G; ++q)
 #pragma unroll
         for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v].re;
     hook();
-    __syncthreads();
+    exchange_sync<T, LOGN, sa, sb>();
 #pragma unroll
     for (int q = 0; q < B::G; ++q)
 #pragma unroll
         for (int v = 0; v < B::N; ++v) r[q * B::N + v].re = rbp[B::off(q, v)];
-    __syncthreads();
+    exchange_sync<T, LOGN, sa, sb>();
 #pragma unroll
     for (int q = 0; q < A::G; ++q)
 #pragma unroll
         for (int v = 0; v < A::N; ++v) wa[A::off(q, v)] = r[q * A::N + v].im;
-    __syncthreads();
+    exchange_sync<T, LOGN, sa, sb>();
 #pragma unroll
     for (int q = 0; q < B::G; ++q)
 #pragma unroll
